@@ -1,0 +1,195 @@
+/*
+ * libmuscle_hip — MI355X (gfx950) backend for Muscle.jl's tensor-contraction
+ * hot path: binary_einsum / binary_einsum! / hadamard / hadamard! /
+ * unary_einsum / unary_einsum!.
+ *
+ * Plain C ABI: every argument is a C scalar, a pointer to plain data or an
+ * opaque handle, so a Julia `ccall`, a Python `ctypes` call or any C caller
+ * maps onto it one to one.  No exception crosses this boundary; every entry
+ * point returns a muscle_hip_status_t and leaves a thread-local message for
+ * muscle_hip_last_error().
+ *
+ * Layout convention (src/Tensor.jl:255-261 of the reference): a tensor is a
+ * base pointer plus, per dimension, an integer mode label, an extent and a
+ * stride counted in ELEMENTS (Julia's `strides(parent(t))`, column-major for a
+ * freshly allocated array, arbitrary for views).  Mode labels are the
+ * integers the Julia side gets from
+ *     Dict(ind => i for (i, ind) in enumerate(unique(inds_a ∪ inds_b)))
+ * exactly as ext/MuscleCUDAExt.jl:24-27 builds them for cuTENSOR.
+ *
+ * Device pointers are HIP device addresses.  All work is enqueued on the
+ * stream set with muscle_hip_set_stream (default: the null stream); calls
+ * return once the work is enqueued.
+ */
+#ifndef MUSCLE_HIP_H
+#define MUSCLE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define MUSCLE_HIP_API __attribute__((visibility("default")))
+#else
+#define MUSCLE_HIP_API
+#endif
+
+/* Status codes.  The Julia/Python wrappers map them onto the reference's
+ * exceptions: INVALID_ARGUMENT -> ArgumentError (binary_einsum.jl:82-83,
+ * hadamard.jl:10), DIM_MISMATCH -> DimensionMismatch (Tensor.jl:16-24),
+ * UNSUPPORTED -> ArgumentError("not implemented", binary_einsum.jl:53-55),
+ * the rest -> ErrorException. */
+typedef enum {
+  MUSCLE_HIP_OK = 0,
+  MUSCLE_HIP_INVALID_ARGUMENT = 1,
+  MUSCLE_HIP_DIM_MISMATCH = 2,
+  MUSCLE_HIP_UNSUPPORTED = 3,
+  MUSCLE_HIP_HIP_ERROR = 4,
+  MUSCLE_HIP_OUT_OF_MEMORY = 5,
+  MUSCLE_HIP_NOT_READY = 6,
+  MUSCLE_HIP_REJECTED = 7
+} muscle_hip_status_t;
+
+/* Element types: Float32, Float64, ComplexF32, ComplexF64 (src/Muscle.jl:46
+ * precompiles Float64/ComplexF64 and their mixed pairs). Complex values are
+ * stored interleaved (re, im) as in Julia. */
+typedef enum {
+  MUSCLE_HIP_F32 = 0,
+  MUSCLE_HIP_F64 = 1,
+  MUSCLE_HIP_C32 = 2,
+  MUSCLE_HIP_C64 = 3
+} muscle_hip_dtype_t;
+
+/* Arithmetic used for binary_einsum.  NATIVE computes in the promoted
+ * element type (FP64 MFMA for Float64/ComplexF64).  The others are the
+ * precision ladder for Float64/ComplexF64 operands; they are accepted for
+ * Float32/ComplexF32 operands too, where FP32 equals NATIVE.
+ *   FP32        operands rounded to FP32, FP32 MFMA, FP32 accumulation.
+ *   BF16X3      each operand split into three bfloat16 terms, the six
+ *               leading cross products on BF16 MFMA, FP32 accumulation per
+ *               K-chunk, chunks summed in FP64.
+ *   FP16X2      two float16 terms, three products, FP32 accumulation per
+ *               K-chunk, chunks summed in FP64.
+ *   OZAKI_INT8  error-free int8 slicing of power-of-two-scaled rows of A and
+ *               columns of B, exact int32 MFMA accumulation, FP64
+ *               recombination; the slice count comes from target_rel_error.
+ */
+typedef enum {
+  MUSCLE_HIP_PRECISION_NATIVE = 0,
+  MUSCLE_HIP_PRECISION_FP32 = 1,
+  MUSCLE_HIP_PRECISION_BF16X3 = 2,
+  MUSCLE_HIP_PRECISION_FP16X2 = 3,
+  MUSCLE_HIP_PRECISION_OZAKI_INT8 = 4
+} muscle_hip_precision_t;
+
+/* One strided tensor operand. */
+typedef struct {
+  int32_t dtype;          /* muscle_hip_dtype_t */
+  int32_t rank;           /* number of dimensions (0 for a scalar) */
+  void* data;             /* device address of element (1,1,...,1) */
+  const int32_t* modes;   /* [rank] mode labels */
+  const int64_t* extents; /* [rank] sizes */
+  const int64_t* strides; /* [rank] strides in elements */
+} muscle_hip_tensor_t;
+
+/* What the library did for one contraction (optional out-parameter). */
+typedef struct {
+  int32_t kernel;        /* 0 none, 1 MFMA tile GEMM, 2 reduction, 3 split-precision, 4 elementwise */
+  int32_t precision;     /* muscle_hip_precision_t actually used */
+  int32_t slices;        /* Ozaki slices per operand (0 otherwise) */
+  int32_t swapped;       /* 1 if A and B were exchanged to make C's fastest mode an N mode */
+  int64_t m, n, k, batch;/* fused GEMM extents */
+  double flops;          /* algorithmic real flops (8*M*N*K*batch for complex) */
+  double bytes;          /* algorithmic bytes: every operand element read or written once */
+  double err_bound;      /* a-priori relative error estimate of the chosen arithmetic */
+} muscle_hip_info_t;
+
+typedef struct muscle_hip_handle_s* muscle_hip_handle_t;
+
+/* --- handle, stream, errors ------------------------------------------- */
+
+/* Creates a handle bound to the current HIP device: plan cache, workspace.
+ * Replaces no reference code: BackendCuTENSOR relies on CUDA.jl's implicit
+ * task-local handle (ext/MuscleCUDAExt.jl:36-38). */
+MUSCLE_HIP_API int muscle_hip_create(muscle_hip_handle_t* handle);
+MUSCLE_HIP_API int muscle_hip_destroy(muscle_hip_handle_t handle);
+/* Sets the hipStream_t all later calls on this handle enqueue on. */
+MUSCLE_HIP_API int muscle_hip_set_stream(muscle_hip_handle_t handle, void* stream);
+MUSCLE_HIP_API int muscle_hip_synchronize(muscle_hip_handle_t handle);
+MUSCLE_HIP_API const char* muscle_hip_status_string(int status);
+/* Detail of the last failing call on this thread. */
+MUSCLE_HIP_API const char* muscle_hip_last_error(void);
+MUSCLE_HIP_API int muscle_hip_version(void);
+/* Number of cached contraction plans, and drop them all. */
+MUSCLE_HIP_API int64_t muscle_hip_plan_cache_size(muscle_hip_handle_t handle);
+MUSCLE_HIP_API int muscle_hip_plan_cache_clear(muscle_hip_handle_t handle);
+
+/* --- the hot path ------------------------------------------------------ */
+
+/* C[modes_c] = sum over contracted modes of A[modes_a] * B[modes_b].
+ *
+ * Replaces binary_einsum(::Backend, inds_c, a, b) (src/Operations/binary_einsum.jl:50,
+ * BackendBase at :76-96) and binary_einsum!(::Backend, c, a, b) (:68, BackendBase
+ * at :98-121); contraction semantics of cuTENSOR.contract! with alpha=1,
+ * beta=0, OP_IDENTITY (ext/MuscleCUDAExt.jl:22-41):
+ *   - a mode in A, B and C is a batch (hyper) mode;
+ *   - a mode in A and B but not C is contracted;
+ *   - a mode of one operand that is absent from C is summed;
+ *   - a mode repeated inside one operand is its diagonal.
+ * C's dtype must be the promotion of A's and B's (MuscleCUDAExt.jl:16).
+ * C is overwritten (beta = 0) and may have any mode order, unlike BackendBase's
+ * binary_einsum! (binary_einsum.jl:108).  `precision` is a
+ * muscle_hip_precision_t; `target_rel_error` (<= 0: default 1e-12) selects the
+ * Ozaki slice count.  `info` may be NULL. */
+MUSCLE_HIP_API int muscle_hip_binary_einsum(muscle_hip_handle_t handle,
+                                            const muscle_hip_tensor_t* c,
+                                            const muscle_hip_tensor_t* a,
+                                            const muscle_hip_tensor_t* b,
+                                            int precision, double target_rel_error,
+                                            muscle_hip_info_t* info);
+
+/* C = A .* broadcast(B), with modes(B) a subset of modes(A) = modes(C), in
+ * any relative order; C has A's element type (copy(a), hadamard.jl:38) and
+ * may alias A (hadamard!(a, a, b), test/unit/operations/hadamard.jl:14-17).
+ * Replaces hadamard(::Backend, a, b) and hadamard!(::Backend, c, a, b)
+ * (src/Operations/hadamard.jl:12,27; BackendBase at :34-77).  A real A with a
+ * complex B is rejected (Julia would raise InexactError on store). */
+MUSCLE_HIP_API int muscle_hip_hadamard(muscle_hip_handle_t handle,
+                                       const muscle_hip_tensor_t* c,
+                                       const muscle_hip_tensor_t* a,
+                                       const muscle_hip_tensor_t* b,
+                                       muscle_hip_info_t* info);
+
+/* Y[modes_y] = sum over the modes of X absent from Y, repeated modes of X
+ * tied (diagonal).  Replaces unary_einsum(::Backend, inds_y, x) and
+ * unary_einsum!(::Backend, y, x) (src/Operations/unary_einsum.jl:35,44), whose
+ * semantics are OMEinsum's einsum!((inds(x),), inds(y), ...)
+ * (ext/MuscleOMEinsumExt.jl:25-38).  Y has X's element type. */
+MUSCLE_HIP_API int muscle_hip_unary_einsum(muscle_hip_handle_t handle,
+                                           const muscle_hip_tensor_t* y,
+                                           const muscle_hip_tensor_t* x,
+                                           muscle_hip_info_t* info);
+
+/* Host-only: lower a binary contraction and report the fused GEMM shape and
+ * the kernel the library would pick, without touching the GPU.  Replaces no
+ * reference function; it exposes the index classification of
+ * binary_einsum.jl:77-83 / MuscleReactantExt.jl:94-122 for inspection.
+ * class_out (may be NULL) receives, per distinct mode in order of first
+ * appearance in (c, a, b): 0 batch, 1 left-open (M), 2 right-open (N),
+ * 3 contracted (K), 4 summed in A only, 5 summed in B only. */
+MUSCLE_HIP_API int muscle_hip_describe(const muscle_hip_tensor_t* c,
+                                       const muscle_hip_tensor_t* a,
+                                       const muscle_hip_tensor_t* b,
+                                       int precision,
+                                       muscle_hip_info_t* info,
+                                       int32_t* modes_out, int32_t* class_out,
+                                       int32_t* nmodes_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUSCLE_HIP_H */

@@ -1,0 +1,537 @@
+// C ABI of libmuscle_hip: handle, plan cache, workspace and the three
+// operations of Muscle.jl's contraction hot path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "handle.hpp"
+#include "kernels.hpp"
+#include "lowering.hpp"
+#include "split.hpp"
+
+namespace mh {
+
+thread_local std::string g_last_error;
+
+namespace {
+
+int64_t product_distinct(const muscle_hip_tensor_t& t) {
+  int64_t p = 1;
+  for (int i = 0; i < t.rank; ++i) {
+    bool seen = false;
+    for (int j = 0; j < i; ++j) seen = seen || t.modes[j] == t.modes[i];
+    if (!seen) p *= t.extents[i];
+  }
+  return p;
+}
+
+// [min, max] element offsets an operand touches.
+void span_of(const muscle_hip_tensor_t& t, int64_t& lo, int64_t& hi) {
+  lo = hi = 0;
+  for (int i = 0; i < t.rank; ++i) {
+    if (t.extents[i] == 0) {
+      lo = 0;
+      hi = -1;
+      return;
+    }
+    const int64_t d = (t.extents[i] - 1) * t.strides[i];
+    if (d < 0) lo += d;
+    else hi += d;
+  }
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+void append_table(std::vector<int64_t>& h, std::vector<size_t>& at, const std::vector<int64_t>& t) {
+  at.push_back(h.size());
+  if (t.empty()) h.push_back(0);
+  else h.insert(h.end(), t.begin(), t.end());
+}
+
+}  // namespace
+
+void Handle::ensure_workspace(size_t bytes) {
+  if (bytes <= ws_size) return;
+  if (ws) {
+    MH_HIP_CHECK(hipStreamSynchronize(stream));
+    MH_HIP_CHECK(hipFree(ws));
+    ws = nullptr;
+    ws_size = 0;
+  }
+  size_t want = std::max(bytes, size_t(1) << 20);
+  if (hipMalloc(&ws, want) != hipSuccess) {
+    (void)hipGetLastError();
+    ws = nullptr;
+    fail(MUSCLE_HIP_OUT_OF_MEMORY, "workspace allocation of " + std::to_string(want) + " bytes failed");
+  }
+  ws_size = want;
+}
+
+Handle::~Handle() {
+  if (ws) (void)hipFree(ws);
+  for (auto& kv : cache)
+    if (kv.second->dtab) (void)hipFree(kv.second->dtab);
+}
+
+Plan* Handle::find_plan(const std::string& key) {
+  auto it = cache.find(key);
+  if (it == cache.end()) return nullptr;
+  lru.splice(lru.begin(), lru, it->second->lru_pos);
+  return it->second.get();
+}
+
+Plan* Handle::insert_plan(const std::string& key, std::unique_ptr<Plan> p) {
+  if (cache.size() >= kMaxPlans) {
+    // evict the least recently used plan; its tables may still be read by
+    // queued kernels, so drain the device first
+    MH_HIP_CHECK(hipDeviceSynchronize());
+    const std::string victim = lru.back();
+    lru.pop_back();
+    auto it = cache.find(victim);
+    if (it != cache.end()) {
+      if (it->second->dtab) (void)hipFree(it->second->dtab);
+      cache.erase(it);
+    }
+  }
+  lru.push_front(key);
+  p->lru_pos = lru.begin();
+  Plan* raw = p.get();
+  cache[key] = std::move(p);
+  return raw;
+}
+
+void upload_tables(Plan& p, hipStream_t s) {
+  const size_t bytes = p.htab.size() * sizeof(int64_t);
+  if (hipMalloc(&p.dtab, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    p.dtab = nullptr;
+    fail(MUSCLE_HIP_OUT_OF_MEMORY, "plan table allocation failed");
+  }
+  MH_HIP_CHECK(hipMemcpyAsync(p.dtab, p.htab.data(), bytes, hipMemcpyHostToDevice, s));
+}
+
+// Builds offset tables and picks a kernel for a lowered contraction.
+std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int precision) {
+  std::unique_ptr<Plan> p(new Plan());
+  p->d = d;
+  p->precision = precision;
+  if (d.empty) {
+    p->kind = Plan::EMPTY;
+    return p;
+  }
+  std::vector<size_t> at;
+  append_table(p->htab, at, offset_table(d.m, 0));      // 0 rowA
+  append_table(p->htab, at, offset_table(d.k, 0));      // 1 kA
+  append_table(p->htab, at, offset_table(d.k, 1));      // 2 kB
+  append_table(p->htab, at, offset_table(d.n, 1));      // 3 colB
+  append_table(p->htab, at, offset_table(d.m, 2));      // 4 rowC
+  append_table(p->htab, at, offset_table(d.n, 2));      // 5 colC
+  append_table(p->htab, at, offset_table(d.batch, 0));  // 6 bA
+  append_table(p->htab, at, offset_table(d.batch, 1));  // 7 bB
+  append_table(p->htab, at, offset_table(d.batch, 2));  // 8 bC
+  p->tab_at = at;
+
+  auto min_stride = [](const std::vector<Dim>& v, int w) {
+    int64_t b = INT64_MAX;
+    for (auto& x : v)
+      if (x.s[w] != 0) b = std::min(b, x.s[w] < 0 ? -x.s[w] : x.s[w]);
+    return b;
+  };
+  p->a_kmajor = min_stride(d.k, 0) < min_stride(d.m, 0);
+  p->b_kmajor = min_stride(d.k, 1) < min_stride(d.n, 1);
+
+  const bool fits32 = d.M < (int64_t(1) << 31) && d.N < (int64_t(1) << 31) && d.K < (int64_t(1) << 31);
+  const bool real_fp = true;
+  if (precision != MUSCLE_HIP_PRECISION_NATIVE && d.has_b && is_double(d.dt[2]) && d.M >= 16 && d.N >= 16 &&
+      d.K >= 16 && fits32) {
+    p->kind = Plan::SPLIT;
+  } else if (d.has_b && real_fp && fits32 && d.M >= 32 && d.N >= 32 && d.K >= 8) {
+    p->kind = Plan::GEMM;
+  } else {
+    p->kind = Plan::REDUCE;
+  }
+  // reduction configuration (also the fallback of the other kinds)
+  const int64_t outputs = d.M * d.N * d.B;
+  int gshift = 0;
+  while (gshift < 6 && (int64_t(8) << gshift) < d.K) ++gshift;
+  int64_t threads = outputs << gshift;
+  int64_t nsplit = 1;
+  const int64_t target = 256 * 1024;
+  if (threads < target && d.K > (int64_t(32) << gshift)) {
+    nsplit = std::min<int64_t>((target + threads - 1) / threads, d.K / (int64_t(32) << gshift));
+    nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, 1024));
+  }
+  p->gshift = gshift;
+  p->nsplit = (int)nsplit;
+  p->kchunk = (d.K + nsplit - 1) / nsplit;
+  if (p->kchunk == 0) p->kchunk = 1;
+  return p;
+}
+
+namespace {
+
+struct Operand {
+  const void* data;
+  int dtype;
+};
+
+// Casts an operand to the compute precision inside the workspace when its
+// precision differs (Float32 with Float64 -> Float64, as Julia promotes).
+const void* promote_operand(Handle& h, const muscle_hip_tensor_t& t, bool want_double, char*& ws_cursor,
+                            int& dtype_out) {
+  dtype_out = t.dtype;
+  if (is_double(t.dtype) == want_double) return t.data;
+  int64_t lo, hi;
+  span_of(t, lo, hi);
+  const int64_t n = hi - lo + 1;
+  const int dst = make_dtype(want_double, is_complex(t.dtype));
+  char* buf = ws_cursor;
+  ws_cursor += align_up((size_t)std::max<int64_t>(n, 1) * dtype_size(dst));
+  const char* src = (const char*)t.data + lo * (int64_t)dtype_size(t.dtype);
+  launch_cast(t.dtype, dst, src, buf, n, h.stream);
+  dtype_out = dst;
+  return buf - lo * (int64_t)dtype_size(dst);
+}
+
+size_t cast_bytes(const muscle_hip_tensor_t& t, bool want_double) {
+  if (is_double(t.dtype) == want_double) return 0;
+  int64_t lo, hi;
+  span_of(t, lo, hi);
+  return align_up((size_t)std::max<int64_t>(hi - lo + 1, 1) * dtype_size(make_dtype(want_double, is_complex(t.dtype))));
+}
+
+void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail(MUSCLE_HIP_HIP_ERROR, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t* c, const muscle_hip_tensor_t* a,
+               const muscle_hip_tensor_t* b) {
+  if (!info) return;
+  std::memset(info, 0, sizeof *info);
+  const ContractionDesc& d = p.d;
+  info->kernel = p.kind == Plan::GEMM ? 1 : p.kind == Plan::REDUCE ? 2 : p.kind == Plan::SPLIT ? 3 : 0;
+  info->precision = p.kind == Plan::SPLIT ? p.precision : MUSCLE_HIP_PRECISION_NATIVE;
+  info->slices = p.slices;
+  info->swapped = d.swapped;
+  info->m = d.M;
+  info->n = d.N;
+  info->k = d.K;
+  info->batch = d.B;
+  const bool ca = is_complex(d.dt[0]), cb = d.has_b && is_complex(d.dt[1]);
+  const double per = d.has_b ? ((ca && cb) ? 8.0 : (ca || cb) ? 4.0 : 2.0) : (ca ? 2.0 : 1.0);
+  info->flops = per * (double)d.M * (double)d.N * (double)d.K * (double)d.B;
+  double bytes = 0;
+  for (const muscle_hip_tensor_t* t : {c, a, b})
+    if (t) bytes += (double)product_distinct(*t) * (double)dtype_size(t->dtype);
+  info->bytes = bytes;
+  info->err_bound = p.err_bound;
+}
+
+}  // namespace
+
+void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const muscle_hip_tensor_t& a,
+                     const muscle_hip_tensor_t* b) {
+  if (p.kind == Plan::EMPTY) return;
+  const ContractionDesc& d = p.d;
+  const bool dbl = is_double(d.dt[2]);
+  // operands in kernel order (after the lowering's optional A/B exchange)
+  const muscle_hip_tensor_t* ka = &a;
+  const muscle_hip_tensor_t* kb = b;
+  if (d.swapped) std::swap(ka, kb);
+  size_t need = cast_bytes(*ka, dbl) + (kb ? cast_bytes(*kb, dbl) : 0);
+  const size_t csize = dtype_size(d.dt[2]);
+  size_t partial_off = need;
+  if (p.kind == Plan::REDUCE && p.nsplit > 1) need += align_up((size_t)(d.M * d.N * d.B) * csize * p.nsplit);
+  size_t split_off = need;
+  if (p.kind == Plan::SPLIT) need += split_workspace_bytes(p);
+  if (need) h.ensure_workspace(need);
+  char* cursor = (char*)h.ws;
+  int dta, dtb = d.dt[1];
+  const void* A = promote_operand(h, *ka, dbl, cursor, dta);
+  const void* B = kb ? promote_operand(h, *kb, dbl, cursor, dtb) : nullptr;
+  const int64_t* T = p.dtab;
+  const auto& at = p.tab_at;
+  if (p.kind == Plan::SPLIT) {
+    run_split(h, p, c.data, A, B, is_complex(dta), is_complex(dtb), (char*)h.ws + split_off);
+    check_launch();
+    return;
+  }
+  if (p.kind == Plan::GEMM) {
+    GemmArgs g;
+    std::memset(&g, 0, sizeof g);
+    g.A = A;
+    g.B = B;
+    g.C = c.data;
+    g.rowA = T + at[0];
+    g.kA = T + at[1];
+    g.kB = T + at[2];
+    g.colB = T + at[3];
+    g.rowC = T + at[4];
+    g.colC = T + at[5];
+    g.M = (int32_t)d.M;
+    g.N = (int32_t)d.N;
+    g.K = (int32_t)d.K;
+    g.a_kmajor = p.a_kmajor;
+    g.b_kmajor = p.b_kmajor;
+    // grid.y carries the batch; split launches beyond the 65535 limit
+    for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
+      g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);
+      g.bA = T + at[6] + b0;
+      g.bB = T + at[7] + b0;
+      g.bC = T + at[8] + b0;
+      launch_gemm_mfma(dbl, is_complex(dta), is_complex(dtb), g, h.stream);
+      check_launch();
+    }
+    return;
+  }
+  ReduceArgs r;
+  std::memset(&r, 0, sizeof r);
+  r.A = A;
+  r.B = B;
+  r.C = c.data;
+  r.partial = (char*)h.ws + partial_off;
+  r.rowA = T + at[0];
+  r.kA = T + at[1];
+  r.kB = T + at[2];
+  r.colB = T + at[3];
+  r.rowC = T + at[4];
+  r.colC = T + at[5];
+  r.bA = T + at[6];
+  r.bB = T + at[7];
+  r.bC = T + at[8];
+  r.M = d.M;
+  r.N = d.N;
+  r.K = d.K;
+  r.batch = d.B;
+  r.outputs = d.M * d.N * d.B;
+  r.kchunk = p.kchunk;
+  r.gshift = p.gshift;
+  r.nsplit = p.nsplit;
+  r.small_index = r.outputs < (int64_t(1) << 32) ? 1 : 0;
+  if (r.small_index) {
+    r.divN = FastDiv((uint32_t)d.N);
+    r.divM = FastDiv((uint32_t)d.M);
+  }
+  launch_reduce(dbl, is_complex(dta), kb && is_complex(dtb), d.has_b, r, h.stream);
+  check_launch();
+}
+
+Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
+               const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b) {
+  std::string key = plan_key(kind, precision, c, a, b);
+  key.append((const char*)&target, sizeof target);
+  Plan* p = h.find_plan(key);
+  if (p) return p;
+  ContractionDesc d = kind == 1 ? lower_binary(*c, *a, *b) : lower_unary(*c, *a);
+  std::unique_ptr<Plan> np = build_contraction_plan(d, precision);
+  if (np->kind == Plan::SPLIT) prepare_split(*np, target);
+  if (np->kind != Plan::EMPTY) upload_tables(*np, h.stream);
+  return h.insert_plan(key, std::move(np));
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return MUSCLE_HIP_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return MUSCLE_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MUSCLE_HIP_INVALID_ARGUMENT;
+  }
+}
+
+extern "C" {
+
+int muscle_hip_version(void) { return 100; }
+
+const char* muscle_hip_status_string(int status) {
+  switch (status) {
+    case MUSCLE_HIP_OK: return "MUSCLE_HIP_OK";
+    case MUSCLE_HIP_INVALID_ARGUMENT: return "MUSCLE_HIP_INVALID_ARGUMENT";
+    case MUSCLE_HIP_DIM_MISMATCH: return "MUSCLE_HIP_DIM_MISMATCH";
+    case MUSCLE_HIP_UNSUPPORTED: return "MUSCLE_HIP_UNSUPPORTED";
+    case MUSCLE_HIP_HIP_ERROR: return "MUSCLE_HIP_HIP_ERROR";
+    case MUSCLE_HIP_OUT_OF_MEMORY: return "MUSCLE_HIP_OUT_OF_MEMORY";
+    case MUSCLE_HIP_NOT_READY: return "MUSCLE_HIP_NOT_READY";
+    case MUSCLE_HIP_REJECTED: return "MUSCLE_HIP_REJECTED";
+  }
+  return "MUSCLE_HIP_UNKNOWN_STATUS";
+}
+
+const char* muscle_hip_last_error(void) { return g_last_error.c_str(); }
+
+int muscle_hip_create(muscle_hip_handle_t* handle) {
+  return guarded([&] {
+    if (!handle) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle pointer");
+    int dev = 0;
+    MH_HIP_CHECK(hipGetDevice(&dev));
+    Handle* h = new Handle();
+    h->device = dev;
+    *handle = reinterpret_cast<muscle_hip_handle_t>(h);
+  });
+}
+
+int muscle_hip_destroy(muscle_hip_handle_t handle) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+  });
+}
+
+int muscle_hip_set_stream(muscle_hip_handle_t handle, void* stream) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->stream = reinterpret_cast<hipStream_t>(stream);
+  });
+}
+
+int muscle_hip_synchronize(muscle_hip_handle_t handle) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    MH_HIP_CHECK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int64_t muscle_hip_plan_cache_size(muscle_hip_handle_t handle) {
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  if (!h) return -1;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return (int64_t)h->cache.size();
+}
+
+int muscle_hip_plan_cache_clear(muscle_hip_handle_t handle) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    MH_HIP_CHECK(hipDeviceSynchronize());
+    for (auto& kv : h->cache)
+      if (kv.second->dtab) (void)hipFree(kv.second->dtab);
+    h->cache.clear();
+    h->lru.clear();
+  });
+}
+
+int muscle_hip_binary_einsum(muscle_hip_handle_t handle, const muscle_hip_tensor_t* c, const muscle_hip_tensor_t* a,
+                             const muscle_hip_tensor_t* b, int precision, double target_rel_error,
+                             muscle_hip_info_t* info) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    check_tensor(c, "C");
+    check_tensor(a, "A");
+    check_tensor(b, "B");
+    if (precision < MUSCLE_HIP_PRECISION_NATIVE || precision > MUSCLE_HIP_PRECISION_OZAKI_INT8)
+      fail(MUSCLE_HIP_INVALID_ARGUMENT, "unknown precision mode");
+    if (!(target_rel_error > 0)) target_rel_error = 1e-12;
+    std::lock_guard<std::mutex> lk(h->mu);
+    Plan* p = get_plan(*h, 1, precision, target_rel_error, c, a, b);
+    run_contraction(*h, *p, *c, *a, b);
+    fill_info(info, *p, c, a, b);
+  });
+}
+
+int muscle_hip_unary_einsum(muscle_hip_handle_t handle, const muscle_hip_tensor_t* y, const muscle_hip_tensor_t* x,
+                            muscle_hip_info_t* info) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    check_tensor(y, "Y");
+    check_tensor(x, "X");
+    std::lock_guard<std::mutex> lk(h->mu);
+    Plan* p = get_plan(*h, 2, MUSCLE_HIP_PRECISION_NATIVE, 0.0, y, x, nullptr);
+    run_contraction(*h, *p, *y, *x, nullptr);
+    fill_info(info, *p, y, x, nullptr);
+  });
+}
+
+int muscle_hip_hadamard(muscle_hip_handle_t handle, const muscle_hip_tensor_t* c, const muscle_hip_tensor_t* a,
+                        const muscle_hip_tensor_t* b, muscle_hip_info_t* info) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    check_tensor(c, "c");
+    check_tensor(a, "a");
+    check_tensor(b, "b");
+    ElementwiseDesc e = lower_hadamard(*c, *a, *b);
+    if ((int)e.dims.size() > kMaxEwDims)
+      fail(MUSCLE_HIP_UNSUPPORTED, "hadamard: more than 24 non-fusable modes");
+    if (info) {
+      std::memset(info, 0, sizeof *info);
+      info->kernel = 4;
+      info->m = e.total;
+      info->n = info->k = info->batch = 1;
+      info->flops = (is_complex(a->dtype) ? (is_complex(b->dtype) ? 6.0 : 2.0) : 1.0) * (double)e.total;
+      info->bytes = (double)e.total * 2 * dtype_size(a->dtype) + (double)product_distinct(*b) * dtype_size(b->dtype);
+    }
+    if (e.total == 0) return;
+    EwArgs w;
+    std::memset(&w, 0, sizeof w);
+    w.C = c->data;
+    w.A = a->data;
+    w.B = b->data;
+    w.total = e.total;
+    w.R = (int32_t)e.dims.size();
+    w.small_index = e.total < (int64_t(1) << 32) ? 1 : 0;
+    for (int i = 0; i < w.R; ++i) {
+      w.ext[i] = e.dims[i].extent;
+      w.sA[i] = e.dims[i].s[0];
+      w.sB[i] = e.dims[i].s[1];
+      w.sC[i] = e.dims[i].s[2];
+      w.div[i] = FastDiv(w.small_index ? (uint32_t)e.dims[i].extent : 1u);
+    }
+    if (w.R == 0) {  // every extent is 1: a single element
+      w.R = 1;
+      w.ext[0] = 1;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    launch_hadamard(a->dtype, b->dtype, w, h->stream);
+    check_launch();
+  });
+}
+
+int muscle_hip_describe(const muscle_hip_tensor_t* c, const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b,
+                        int precision, muscle_hip_info_t* info, int32_t* modes_out, int32_t* class_out,
+                        int32_t* nmodes_out) {
+  return guarded([&] {
+    check_tensor(c, "C", false);
+    check_tensor(a, "A", false);
+    check_tensor(b, "B", false);
+    ContractionDesc d = lower_binary(*c, *a, *b);
+    std::unique_ptr<Plan> p = build_contraction_plan(d, precision);
+    if (p->kind == Plan::SPLIT) prepare_split(*p, 1e-12);
+    fill_info(info, *p, c, a, b);
+    if (nmodes_out) *nmodes_out = (int32_t)d.modes.size();
+    for (size_t i = 0; i < d.modes.size(); ++i) {
+      if (modes_out) modes_out[i] = d.modes[i].label;
+      if (class_out) class_out[i] = d.modes[i].cls;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// Device-side element types and arithmetic helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mh {
+
+template <typename T>
+struct Cplx {
+  T re, im;
+};
+
+// Storage type of one element: T for real, a 2-vector for complex so a
+// complex element is one 8- or 16-byte memory access.
+template <typename T, bool C>
+struct Elem;
+template <>
+struct Elem<double, false> {
+  typedef double type;
+};
+template <>
+struct Elem<float, false> {
+  typedef float type;
+};
+template <>
+struct Elem<double, true> {
+  typedef double2 type;
+};
+template <>
+struct Elem<float, true> {
+  typedef float2 type;
+};
+
+template <typename T, bool C>
+__device__ inline Cplx<T> load_elem(const void* base, int64_t off) {
+  if constexpr (C) {
+    auto v = reinterpret_cast<const typename Elem<T, true>::type*>(base)[off];
+    return Cplx<T>{v.x, v.y};
+  } else {
+    return Cplx<T>{reinterpret_cast<const T*>(base)[off], T(0)};
+  }
+}
+
+template <typename T, bool C>
+__device__ inline void store_elem(void* base, int64_t off, Cplx<T> v) {
+  if constexpr (C) {
+    typename Elem<T, true>::type w;
+    w.x = v.re;
+    w.y = v.im;
+    reinterpret_cast<typename Elem<T, true>::type*>(base)[off] = w;
+  } else {
+    reinterpret_cast<T*>(base)[off] = v.re;
+  }
+}
+
+}  // namespace mh

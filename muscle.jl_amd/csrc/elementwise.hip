@@ -1,0 +1,122 @@
+// Broadcast elementwise product (hadamard) and dtype casts.
+//
+// hadamard: C = A .* broadcast(B) (src/Operations/hadamard.jl:34-77).  The
+// reference first permutes B to A's relative mode order and reshapes it to a
+// broadcast shape (:59-70); here B simply has stride 0 along the modes it
+// lacks and its own strides along the rest, so no permuted copy is made and
+// any relative order of B's modes is handled (the reference's findperm call
+// at :68-69 is only right for involutive orders).  Modes are fused on the
+// host, so the common same-layout case is a single dimension.  C has A's
+// element type (copy(a), hadamard.jl:38); the product is formed in the
+// promoted type and converted on store.
+#include "device_types.hpp"
+#include "kernels.hpp"
+#include "muscle_hip.h"
+
+namespace mh {
+namespace {
+
+constexpr int NT = 256;
+
+template <typename TA, bool CA, typename TB, bool CB>
+__global__ __launch_bounds__(NT) void hadamard_kernel(EwArgs p) {
+  typedef decltype(TA(0) * TB(0)) TP;  // promoted real type
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t o = (int64_t)blockIdx.x * NT + threadIdx.x; o < p.total; o += stride) {
+    int64_t offA = 0, offB = 0, offC = 0;
+    if (p.R == 1) {
+      offA = o * p.sA[0];
+      offB = o * p.sB[0];
+      offC = o * p.sC[0];
+    } else if (p.small_index) {
+      uint32_t rem = (uint32_t)o;
+      for (int d = 0; d < p.R; ++d) {
+        const uint32_t q = p.div[d].div(rem);
+        const int64_t i = rem - q * (uint32_t)p.ext[d];
+        offA += i * p.sA[d];
+        offB += i * p.sB[d];
+        offC += i * p.sC[d];
+        rem = q;
+      }
+    } else {
+      int64_t rem = o;
+      for (int d = 0; d < p.R; ++d) {
+        const int64_t q = rem / p.ext[d];
+        const int64_t i = rem - q * p.ext[d];
+        offA += i * p.sA[d];
+        offB += i * p.sB[d];
+        offC += i * p.sC[d];
+        rem = q;
+      }
+    }
+    const Cplx<TA> a = load_elem<TA, CA>(p.A, offA);
+    const Cplx<TB> b = load_elem<TB, CB>(p.B, offB);
+    const TP ar = a.re, ai = a.im, br = b.re, bi = b.im;
+    Cplx<TA> c;
+    if constexpr (CA && CB) {
+      c.re = (TA)(ar * br - ai * bi);
+      c.im = (TA)(ar * bi + ai * br);
+    } else if constexpr (CA) {
+      c.re = (TA)(ar * br);
+      c.im = (TA)(ai * br);
+    } else {
+      c.re = (TA)(ar * br);
+      c.im = TA(0);
+    }
+    store_elem<TA, CA>(p.C, offC, c);
+  }
+}
+
+template <typename TA, bool CA, typename TB, bool CB>
+void launch_h(const EwArgs& e, hipStream_t s) {
+  int64_t blocks = (e.total + NT - 1) / NT;
+  if (blocks > 65536) blocks = 65536;  // grid-stride beyond ~16M threads
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((hadamard_kernel<TA, CA, TB, CB>), dim3((unsigned)blocks), dim3(NT), 0, s, e);
+}
+
+template <typename TS, typename TD, bool C>
+__global__ __launch_bounds__(NT) void cast_kernel(const void* src, void* dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
+    const Cplx<TS> v = load_elem<TS, C>(src, i);
+    store_elem<TD, C>(dst, i, Cplx<TD>{(TD)v.re, (TD)v.im});
+  }
+}
+
+}  // namespace
+
+void launch_hadamard(int dtA, int dtB, const EwArgs& e, hipStream_t s) {
+  switch (dtA * 4 + dtB) {
+    case MUSCLE_HIP_F32 * 4 + MUSCLE_HIP_F32: launch_h<float, false, float, false>(e, s); break;
+    case MUSCLE_HIP_F32 * 4 + MUSCLE_HIP_F64: launch_h<float, false, double, false>(e, s); break;
+    case MUSCLE_HIP_F64 * 4 + MUSCLE_HIP_F32: launch_h<double, false, float, false>(e, s); break;
+    case MUSCLE_HIP_F64 * 4 + MUSCLE_HIP_F64: launch_h<double, false, double, false>(e, s); break;
+    case MUSCLE_HIP_C32 * 4 + MUSCLE_HIP_F32: launch_h<float, true, float, false>(e, s); break;
+    case MUSCLE_HIP_C32 * 4 + MUSCLE_HIP_F64: launch_h<float, true, double, false>(e, s); break;
+    case MUSCLE_HIP_C32 * 4 + MUSCLE_HIP_C32: launch_h<float, true, float, true>(e, s); break;
+    case MUSCLE_HIP_C32 * 4 + MUSCLE_HIP_C64: launch_h<float, true, double, true>(e, s); break;
+    case MUSCLE_HIP_C64 * 4 + MUSCLE_HIP_F32: launch_h<double, true, float, false>(e, s); break;
+    case MUSCLE_HIP_C64 * 4 + MUSCLE_HIP_F64: launch_h<double, true, double, false>(e, s); break;
+    case MUSCLE_HIP_C64 * 4 + MUSCLE_HIP_C32: launch_h<double, true, float, true>(e, s); break;
+    case MUSCLE_HIP_C64 * 4 + MUSCLE_HIP_C64: launch_h<double, true, double, true>(e, s); break;
+    default: break;  // real a with complex b is rejected on the host
+  }
+}
+
+void launch_cast(int dt_src, int dt_dst, const void* src, void* dst, int64_t n, hipStream_t s) {
+  int64_t blocks = (n + NT - 1) / NT;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  const dim3 g((unsigned)blocks), b(NT);
+  if (dt_src == MUSCLE_HIP_F32 && dt_dst == MUSCLE_HIP_F64)
+    hipLaunchKernelGGL((cast_kernel<float, double, false>), g, b, 0, s, src, dst, n);
+  else if (dt_src == MUSCLE_HIP_F64 && dt_dst == MUSCLE_HIP_F32)
+    hipLaunchKernelGGL((cast_kernel<double, float, false>), g, b, 0, s, src, dst, n);
+  else if (dt_src == MUSCLE_HIP_C32 && dt_dst == MUSCLE_HIP_C64)
+    hipLaunchKernelGGL((cast_kernel<float, double, true>), g, b, 0, s, src, dst, n);
+  else if (dt_src == MUSCLE_HIP_C64 && dt_dst == MUSCLE_HIP_C32)
+    hipLaunchKernelGGL((cast_kernel<double, float, true>), g, b, 0, s, src, dst, n);
+}
+
+}  // namespace mh

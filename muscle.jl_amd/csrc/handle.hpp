@@ -1,0 +1,64 @@
+// Library handle and cached contraction plans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "lowering.hpp"
+
+namespace mh {
+
+// A lowered contraction with its offset tables resident on the device.
+struct Plan {
+  enum Kind { EMPTY = 0, GEMM = 1, REDUCE = 2, SPLIT = 3 };
+  int kind = EMPTY;
+  int precision = MUSCLE_HIP_PRECISION_NATIVE;
+  ContractionDesc d;
+  // offset tables: rowA kA kB colB rowC colC bA bB bC, concatenated
+  std::vector<int64_t> htab;
+  std::vector<size_t> tab_at;
+  int64_t* dtab = nullptr;
+  bool a_kmajor = false, b_kmajor = false;
+  // reduction kernel configuration
+  int gshift = 0;
+  int nsplit = 1;
+  int64_t kchunk = 1;
+  // split-precision configuration
+  int slices = 0;
+  int64_t kblock = 0;  // K-chunk whose FP32 partial sums are exact / combined in FP64
+  double err_bound = 0.0;
+  std::list<std::string>::iterator lru_pos;
+};
+
+struct Handle {
+  static constexpr size_t kMaxPlans = 8192;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::unordered_map<std::string, std::unique_ptr<Plan>> cache;
+  std::list<std::string> lru;
+  void* ws = nullptr;
+  size_t ws_size = 0;
+
+  ~Handle();
+  void ensure_workspace(size_t bytes);
+  Plan* find_plan(const std::string& key);
+  Plan* insert_plan(const std::string& key, std::unique_ptr<Plan> p);
+};
+
+extern thread_local std::string g_last_error;
+
+std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int precision);
+void upload_tables(Plan& p, hipStream_t s);
+void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const muscle_hip_tensor_t& a,
+                     const muscle_hip_tensor_t* b);
+Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
+               const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b);
+
+}  // namespace mh

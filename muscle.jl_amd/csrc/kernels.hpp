@@ -1,0 +1,99 @@
+// Launch interface between the host layer and the HIP kernels.  Offsets are
+// in elements of the operand's own type; every table is a device array.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mh {
+
+// Unsigned 32-bit division by a run-time constant with a multiply-high and a
+// shift: the round-up method of Granlund & Montgomery, "Division by invariant
+// integers using multiplication" (PLDI 1994).  Valid for any n < 2^32.
+struct FastDiv {
+  uint32_t d, mul, shift;
+  FastDiv() : d(1), mul(1), shift(0) {}
+  explicit FastDiv(uint32_t div) : d(div) {
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < div) ++l;
+    shift = l;
+    mul = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - div)) / div + 1);
+  }
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t t = __umulhi(mul, n);
+#else
+    uint32_t t = (uint32_t)(((uint64_t)mul * n) >> 32);
+#endif
+    return (uint32_t)(((uint64_t)t + n) >> shift);
+  }
+};
+
+// ---- tile GEMM on MFMA ----------------------------------------------------
+struct GemmArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  const int64_t* rowA;  // [M]
+  const int64_t* kA;    // [K]
+  const int64_t* kB;    // [K]
+  const int64_t* colB;  // [N]
+  const int64_t* rowC;  // [M]
+  const int64_t* colC;  // [N]
+  const int64_t* bA;    // [batch]
+  const int64_t* bB;
+  const int64_t* bC;
+  int32_t M, N, K, batch;
+  int32_t a_kmajor;  // A's contiguous direction is K (else M)
+  int32_t b_kmajor;  // B's contiguous direction is K (else N)
+  int32_t tiles_m, tiles_n;
+};
+// dbl: FP64 (else FP32) arithmetic; ca/cb: operand is complex.
+void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g, hipStream_t s);
+
+// ---- reduction kernel (skinny contractions, unary sums, traces) -------------
+struct ReduceArgs {
+  const void* A;
+  const void* B;  // null when has_b == false
+  void* C;
+  void* partial;  // [nsplit][outputs] when nsplit > 1
+  const int64_t* rowA;
+  const int64_t* kA;
+  const int64_t* kB;
+  const int64_t* colB;
+  const int64_t* rowC;
+  const int64_t* colC;
+  const int64_t* bA;
+  const int64_t* bB;
+  const int64_t* bC;
+  int64_t M, N, K, batch, outputs;
+  int64_t kchunk;
+  int32_t gshift;  // lanes per output = 1 << gshift (<= 64)
+  int32_t nsplit;
+  FastDiv divN, divM;  // valid when outputs < 2^32
+  int32_t small_index;  // 1: use FastDiv
+};
+void launch_reduce(bool dbl, bool ca, bool cb, bool has_b, const ReduceArgs& r, hipStream_t s);
+
+// ---- elementwise broadcast product (hadamard) --------------------------------
+constexpr int kMaxEwDims = 24;
+struct EwArgs {
+  void* C;
+  const void* A;
+  const void* B;
+  int64_t total;
+  int32_t R;
+  int32_t small_index;
+  FastDiv div[kMaxEwDims];
+  int64_t ext[kMaxEwDims];
+  int64_t sA[kMaxEwDims], sB[kMaxEwDims], sC[kMaxEwDims];
+};
+// dtA/dtB: muscle_hip_dtype_t of a and b; c has a's type.
+void launch_hadamard(int dtA, int dtB, const EwArgs& e, hipStream_t s);
+
+// ---- dtype cast over a strided span (mixed-precision operands) --------------
+// dst[i] = (dst type) src[i] for i in [0, n); complexness is preserved.
+void launch_cast(int dt_src, int dt_dst, const void* src, void* dst, int64_t n, hipStream_t s);
+
+}  // namespace mh

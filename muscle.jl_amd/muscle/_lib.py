@@ -1,0 +1,156 @@
+"""ctypes binding of libmuscle_hip.so (declared in include/muscle_hip.h).
+
+The library is the product: there is no CPU fallback.  If the shared
+library is missing this module raises ImportError at load time; if no GPU is
+present, creating a handle fails with the HIP error the runtime reports.
+torch is imported first so that its HIP runtime (libamdhip64.so.7) is the
+one the library binds to — device pointers from torch's allocator and the
+library's kernels then live in one runtime.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (loads the HIP runtime before the library)
+
+from .errors import ArgumentError, DimensionMismatch, MuscleHipError, UnsupportedError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libmuscle_hip.so"))
+
+# status codes (muscle_hip_status_t)
+OK, INVALID_ARGUMENT, DIM_MISMATCH, UNSUPPORTED, HIP_ERROR, OUT_OF_MEMORY, NOT_READY, REJECTED = range(8)
+# dtypes (muscle_hip_dtype_t)
+F32, F64, C32, C64 = range(4)
+# precision modes (muscle_hip_precision_t)
+PRECISION = {"native": 0, "fp32": 1, "bf16x3": 2, "fp16x2": 3, "ozaki_int8": 4}
+
+DTYPE_CODE = {torch.float32: F32, torch.float64: F64, torch.complex64: C32, torch.complex128: C64}
+CODE_DTYPE = {v: k for k, v in DTYPE_CODE.items()}
+
+
+class TensorDesc(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int32),
+        ("rank", ctypes.c_int32),
+        ("data", ctypes.c_void_p),
+        ("modes", ctypes.POINTER(ctypes.c_int32)),
+        ("extents", ctypes.POINTER(ctypes.c_int64)),
+        ("strides", ctypes.POINTER(ctypes.c_int64)),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("slices", ctypes.c_int32),
+        ("swapped", ctypes.c_int32),
+        ("m", ctypes.c_int64),
+        ("n", ctypes.c_int64),
+        ("k", ctypes.c_int64),
+        ("batch", ctypes.c_int64),
+        ("flops", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+        ("err_bound", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libmuscle_hip.so not found at {LIB_PATH}; build it with `make -C muscle.jl_amd/csrc` "
+            "(the HIP backend has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    H = ctypes.c_void_p
+    T = ctypes.POINTER(TensorDesc)
+    I = ctypes.POINTER(Info)
+    sig = {
+        "muscle_hip_version": ([], ctypes.c_int),
+        "muscle_hip_create": ([ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "muscle_hip_destroy": ([H], ctypes.c_int),
+        "muscle_hip_set_stream": ([H, ctypes.c_void_p], ctypes.c_int),
+        "muscle_hip_synchronize": ([H], ctypes.c_int),
+        "muscle_hip_status_string": ([ctypes.c_int], ctypes.c_char_p),
+        "muscle_hip_last_error": ([], ctypes.c_char_p),
+        "muscle_hip_plan_cache_size": ([H], ctypes.c_int64),
+        "muscle_hip_plan_cache_clear": ([H], ctypes.c_int),
+        "muscle_hip_binary_einsum": ([H, T, T, T, ctypes.c_int, ctypes.c_double, I], ctypes.c_int),
+        "muscle_hip_hadamard": ([H, T, T, T, I], ctypes.c_int),
+        "muscle_hip_unary_einsum": ([H, T, T, I], ctypes.c_int),
+        "muscle_hip_describe": ([T, T, T, ctypes.c_int, I, ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    """Raise the Julia-side exception a status code maps to."""
+    if status == OK:
+        return
+    msg = (lib.muscle_hip_last_error() or b"").decode(errors="replace")
+    name = lib.muscle_hip_status_string(status).decode()
+    if status == INVALID_ARGUMENT:
+        raise ArgumentError(msg)
+    if status == DIM_MISMATCH:
+        raise DimensionMismatch(msg)
+    if status == UNSUPPORTED:
+        raise UnsupportedError(msg)
+    raise MuscleHipError(f"{name}: {msg}", status)
+
+
+class _Handles(threading.local):
+    def __init__(self):
+        self.by_device = {}
+
+
+_handles = _Handles()
+
+
+def handle(device: int) -> ctypes.c_void_p:
+    """The calling thread's library handle for a device (created on first use)."""
+    h = _handles.by_device.get(device)
+    if h is None:
+        with torch.cuda.device(device):
+            h = ctypes.c_void_p()
+            check(lib.muscle_hip_create(ctypes.byref(h)))
+        _handles.by_device[device] = h
+    return h
+
+
+class Desc:
+    """Keeps a TensorDesc and the arrays it points into alive for one call."""
+
+    __slots__ = ("desc", "_keep")
+
+    def __init__(self, dtype: int, data_ptr: int, modes, extents, strides):
+        r = len(modes)
+        m = (ctypes.c_int32 * max(r, 1))(*modes)
+        e = (ctypes.c_int64 * max(r, 1))(*extents)
+        s = (ctypes.c_int64 * max(r, 1))(*strides)
+        self._keep = (m, e, s)
+        self.desc = TensorDesc(dtype, r, ctypes.c_void_p(data_ptr),
+                               ctypes.cast(m, ctypes.POINTER(ctypes.c_int32)),
+                               ctypes.cast(e, ctypes.POINTER(ctypes.c_int64)),
+                               ctypes.cast(s, ctypes.POINTER(ctypes.c_int64)))
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.desc)
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C ABI entry points bound here."""
+    return [n for n in dir(lib) if n.startswith("muscle_hip_")]

@@ -1,0 +1,103 @@
+"""Synthetic workloads of the contraction hot path (shapes and seeded inputs).
+
+  config1  plain ComplexF64 binary_einsum: A(i,j,k) = (64,32,48),
+           B(k,l,j) = (48,96,32), dims = [j,k] -> C(i,l) 64x96; real and
+           imaginary parts iid N(0,1)/sqrt(1536).
+  config2  rank-6 tensors with every extent 16, contracted modes at positions
+           (2,4,6) of A and (1,3,5) of B, output (a1,b1,a3,b2,a5,b3):
+           M = N = K = 4096 on interleaved layouts; entries CN(0,1)/sqrt(K).
+  config2h hyperindex variant: batch 64 with M = N = K = 512 per batch.
+
+Inputs are column-major like Julia arrays.  The numpy generators draw, for
+each operand in turn, all real parts then all imaginary parts from
+numpy.random.default_rng(seed).standard_normal, in column-major order.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def _cn(rng, shape, scale, dtype=np.complex128):
+    n = int(np.prod(shape))
+    re = rng.standard_normal(n)
+    im = rng.standard_normal(n)
+    z = (re + 1j * im) * scale
+    return np.reshape(z.astype(dtype), shape, order="F")
+
+
+def config1_inputs(seed: int = 0):
+    """(A, inds_a, B, inds_b, dims, inds_c) for the plain ComplexF64 case."""
+    rng = np.random.default_rng(seed)
+    s = 1.0 / math.sqrt(1536)
+    A = _cn(rng, (64, 32, 48), s)
+    B = _cn(rng, (48, 96, 32), s)
+    return A, ["i", "j", "k"], B, ["k", "l", "j"], ["j", "k"], ["i", "l"]
+
+
+CONFIG2_A = ["a1", "c1", "a3", "c2", "a5", "c3"]
+CONFIG2_B = ["c1", "b1", "c2", "b2", "c3", "b3"]
+CONFIG2_C = ["a1", "b1", "a3", "b2", "a5", "b3"]
+
+
+def config2_shapes(d: int = 16):
+    return [d] * 6, [d] * 6, [d] * 6
+
+
+def config2_inputs(d: int = 16, seed: int = 0, dtype=np.complex128):
+    """Rank-6 interleaved case at extent d per mode (d = 16: M = N = K = 4096)."""
+    rng = np.random.default_rng(seed)
+    K = d ** 3
+    s = 1.0 / math.sqrt(2 * K)  # CN(0,1): E|z|^2 = 1
+    A = _cn(rng, (d,) * 6, s, dtype)
+    B = _cn(rng, (d,) * 6, s, dtype)
+    return A, list(CONFIG2_A), B, list(CONFIG2_B), list(CONFIG2_C)
+
+
+CONFIG2H_A = ["a1", "c1", "h", "a2", "c2"]
+CONFIG2H_B = ["c1", "b1", "h", "c2", "b2"]
+CONFIG2H_C = ["a1", "b1", "h", "a2", "b2"]
+
+
+def config2h_sizes(scale: int = 1):
+    """Extents of the hyperindex case; scale = 1 gives batch 64, M = N = K = 512."""
+    return {"a1": 16 // scale, "c1": 16 // scale, "h": 64 // scale, "a2": 32 // scale, "c2": 32 // scale,
+            "b1": 16 // scale, "b2": 32 // scale}
+
+
+def config2h_inputs(scale: int = 1, seed: int = 0, dtype=np.complex128):
+    sz = config2h_sizes(scale)
+    rng = np.random.default_rng(seed)
+    K = sz["c1"] * sz["c2"]
+    s = 1.0 / math.sqrt(2 * K)
+    A = _cn(rng, tuple(sz[i] for i in CONFIG2H_A), s, dtype)
+    B = _cn(rng, tuple(sz[i] for i in CONFIG2H_B), s, dtype)
+    return A, list(CONFIG2H_A), B, list(CONFIG2H_B), list(CONFIG2H_C)
+
+
+# ------------------------------------------------------------ precision-ladder distributions
+def ladder_inputs(dist: str, seed: int = 0):
+    """config-1 shapes under three value distributions:
+    "normal"     iid N(0,1) real and imaginary parts;
+    "loguniform" |z| log-uniform in [1e-4, 1e4], phase uniform;
+    "unitrows"   iid normal, then every row of A's matrix view (A[i, (j,k)]) and
+                 every column of B's ((k,j), l) scaled to unit 2-norm."""
+    rng = np.random.default_rng(seed)
+    sa, sb = (64, 32, 48), (48, 96, 32)
+    if dist == "normal":
+        A, B = _cn(rng, sa, 1.0), _cn(rng, sb, 1.0)
+    elif dist == "loguniform":
+        def lu(shape):
+            n = int(np.prod(shape))
+            mag = 10.0 ** rng.uniform(-4, 4, n)
+            ph = rng.uniform(0, 2 * np.pi, n)
+            return np.reshape(mag * np.exp(1j * ph), shape, order="F")
+        A, B = lu(sa), lu(sb)
+    elif dist == "unitrows":
+        A, B = _cn(rng, sa, 1.0), _cn(rng, sb, 1.0)
+        A = A / np.sqrt(np.sum(np.abs(A) ** 2, axis=(1, 2), keepdims=True))
+        B = B / np.sqrt(np.sum(np.abs(B) ** 2, axis=(0, 2), keepdims=True))
+    else:
+        raise ValueError(dist)
+    return A, ["i", "j", "k"], B, ["k", "l", "j"], ["j", "k"], ["i", "l"]

@@ -1,0 +1,191 @@
+"""binary_einsum / binary_einsum! on the GPU against the CPU oracle.
+
+Bar: exact for the reference's `ones` known answers; norm-wise relative error
+<= 1e-12 for Float64/ComplexF64 (the FP64 MFMA path), <= 2e-5 for the FP32
+path, on seeded inputs.  Full-size config-2 results are checked on sampled
+entries against the oracle's dot products and through a size-independent
+identity (sum over n of C equals A times the row sums of B).
+"""
+import zlib
+
+import numpy as np
+import pytest
+import torch
+
+import muscle
+from oracle import backend_base as bb
+from tests import kats
+from tests.gpu_util import TOL, dev_tensor, host, randn, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(A, ia, B, ib, **kw):
+    a, b = dev_tensor(A, ia), dev_tensor(B, ib)
+    c = muscle.binary_einsum(a, b, **kw)
+    return c, host(c)
+
+
+# ------------------------------------------------------------ known answers
+@pytest.mark.parametrize("case", kats.binary_kats(), ids=lambda c: c[0])
+def test_binary_kats(case):
+    name, A, ia, B, ib, kw, out_inds, val, base_ok = case
+    c, C = _run(A, ia, B, ib, **kw)
+    assert [i.tag for i in c.inds()] == out_inds
+    assert C.shape == val.shape
+    assert np.array_equal(C, val), (name, C, val)
+
+
+@pytest.mark.parametrize("T", [np.float64, np.complex128])
+def test_manual_kat(T):
+    A, ia, B, ib = kats.manual_case(T)
+    c, C = _run(A, ia, B, ib, dims=["j", "k"])
+    assert [i.tag for i in c.inds()] == ["i", "l"]
+    assert np.allclose(C, kats.manual_expected_full(A, B))
+    c, C = _run(A, ia, B, ib, dims=["j"])
+    assert [i.tag for i in c.inds()] == ["i", "k", "l"]
+    assert np.allclose(C, kats.manual_expected_hyper(A, B))
+
+
+# ------------------------------------------------------------ seeded parity
+CASES = [
+    # name, inds_a, shape_a, inds_b, shape_b, inds_c
+    ("config1", ["i", "j", "k"], (64, 32, 48), ["k", "l", "j"], (48, 96, 32), ["i", "l"]),
+    ("config1 out^T", ["i", "j", "k"], (64, 32, 48), ["k", "l", "j"], (48, 96, 32), ["l", "i"]),
+    ("ragged", ["i", "j"], (67, 45), ["j", "k"], (45, 93), ["i", "k"]),
+    ("ragged2", ["i", "j"], (33, 17), ["k", "j"], (129, 17), ["k", "i"]),
+    ("k-major A", ["j", "i"], (40, 70), ["j", "k"], (40, 50), ["i", "k"]),
+    ("interleaved d=4", ["a1", "c1", "a3", "c2", "a5", "c3"], (4,) * 6, ["c1", "b1", "c2", "b2", "c3", "b3"], (4,) * 6,
+     ["a1", "b1", "a3", "b2", "a5", "b3"]),
+    ("interleaved d=8", ["a1", "c1", "a3", "c2", "a5", "c3"], (8,) * 6, ["c1", "b1", "c2", "b2", "c3", "b3"], (8,) * 6,
+     ["a1", "b1", "a3", "b2", "a5", "b3"]),
+    ("hyper", ["a1", "c1", "h", "a2", "c2"], (4, 4, 8, 8, 8), ["c1", "b1", "h", "c2", "b2"], (4, 4, 8, 8, 8),
+     ["a1", "b1", "h", "a2", "b2"]),
+    ("hyper batch-first out", ["i", "j", "h"], (40, 36, 3), ["j", "k", "h"], (36, 50, 3), ["h", "k", "i"]),
+    ("matvec", ["i", "j"], (300, 200), ["j"], (200,), ["i"]),
+    ("vecmat", ["j"], (200,), ["j", "k"], (200, 300), ["k"]),
+    ("inner", ["i", "j"], (120, 130), ["j", "i"], (130, 120), []),
+    ("outer", ["i", "j"], (20, 30), ["k"], (40,), ["k", "i", "j"]),
+    ("dims 2 tensors", ["a", "b", "c", "d", "e"], (2,) * 5, ["c", "d", "e", "f", "g"], (2,) * 5, ["g", "a", "f", "b"]),
+    ("single-operand sum", ["i", "j", "s"], (30, 40, 5), ["j", "k"], (40, 35), ["i", "k"]),
+    ("diagonal operand", ["i", "j", "i"], (20, 30, 20), ["j", "k"], (30, 40), ["i", "k"]),
+]
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.float64, np.complex64, np.float32])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_binary_parity(case, T):
+    name, ia, sa, ib, sb, ic = case
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    A, B = randn(rng, sa, T), randn(rng, sb, T)
+    ref = bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib)
+    c, C = _run(A, ia, B, ib, out=ic)
+    assert C.dtype == T
+    assert rel_err(C, ref) <= TOL[T], (name, rel_err(C, ref))
+
+
+def test_config1_golden():
+    """Config 1 against the committed golden output (tests/golden/config1_c64.npz)."""
+    import os
+    from muscle.workloads import config1_inputs
+    A, ia, B, ib, dims, ic = config1_inputs(0)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "config1_c64.npz"))
+    assert np.allclose(A.ravel(order="F")[:8], g["a_head"], rtol=0, atol=0)
+    c, C = _run(A, ia, B, ib, dims=dims)
+    assert [i.tag for i in c.inds()] == ic
+    err = rel_err(C, g["c"])
+    assert err <= 1e-12, err
+    info = muscle.last_info()
+    assert info["kernel"] == 1 and info["flops"] == 75497472.0
+
+
+@pytest.mark.parametrize("ta, tb", [(np.float64, np.complex128), (np.complex128, np.float64),
+                                    (np.float32, np.float64), (np.float32, np.complex64),
+                                    (np.complex64, np.float64)])
+def test_mixed_eltypes_promote(ta, tb):
+    rng = np.random.default_rng(4)
+    A, B = randn(rng, (50, 40), ta), randn(rng, (40, 60), tb)
+    c, C = _run(A, ["i", "j"], B, ["j", "k"])
+    T = np.result_type(ta, tb)
+    assert C.dtype == T
+    ref = bb.binary_einsum_base(["i", "k"], A.astype(T), ["i", "j"], B.astype(T), ["j", "k"])
+    assert rel_err(C, ref) <= TOL[T.type]
+
+
+def test_strided_views():
+    rng = np.random.default_rng(8)
+    big = randn(rng, (80, 90), np.complex128)
+    Bn = randn(rng, (45, 70), np.complex128)
+    at = torch.from_numpy(np.asfortranarray(big)).cuda()[::2, 1::2]  # (40, 45) view, strides (2, 160)
+    a = muscle.Tensor(at, ["i", "j"])
+    b = dev_tensor(Bn, ["j", "k"])
+    c = muscle.binary_einsum(a, b)
+    ref = big[::2, 1::2] @ Bn
+    assert rel_err(host(c), ref) <= 1e-12
+
+
+def test_inplace_any_output_order():
+    rng = np.random.default_rng(12)
+    A, B = randn(rng, (30, 20, 10)), randn(rng, (10, 25, 20))
+    a, b = dev_tensor(A, ["i", "j", "k"]), dev_tensor(B, ["k", "l", "j"])
+    c = muscle.Tensor(torch.zeros((25, 30), dtype=torch.complex128, device="cuda"), ["l", "i"])
+    out = muscle.binary_einsum_(c, a, b)
+    assert out is c
+    ref = bb.binary_einsum_base(["l", "i"], A, ["i", "j", "k"], B, ["k", "l", "j"])
+    assert rel_err(host(c), ref) <= 1e-12
+
+
+def test_empty_and_zero_extents():
+    a = dev_tensor(np.ones((0, 3)), ["i", "j"])
+    b = dev_tensor(np.ones((3, 4)), ["j", "k"])
+    c = muscle.binary_einsum(a, b)
+    assert c.shape == (0, 4)
+    a = dev_tensor(np.ones((2, 0)), ["i", "j"])
+    b = dev_tensor(np.ones((0, 4)), ["j", "k"])
+    c = muscle.binary_einsum(a, b)
+    assert np.array_equal(host(c), np.zeros((2, 4)))  # sum over an empty mode
+
+
+def test_full_contraction_large_k():
+    """Inner product to a 0-dim tensor over 2^22 elements: the split-K path."""
+    rng = np.random.default_rng(13)
+    A = randn(rng, (2048, 2048), np.complex128)
+    B = randn(rng, (2048, 2048), np.complex128)
+    c, C = _run(A, ["i", "j"], B, ["i", "j"])
+    ref = np.sum(A * B)
+    assert abs(C - ref) / abs(ref) < 1e-12
+
+
+def test_errors_map_to_julia_exceptions():
+    a = dev_tensor(np.ones((2, 3)), ["i", "j"])
+    b = dev_tensor(np.ones((4, 5)), ["j", "k"])
+    with pytest.raises(muscle.DimensionMismatch):
+        muscle.binary_einsum(a, b)
+    b = dev_tensor(np.ones((3, 5)), ["j", "k"])
+    c = muscle.Tensor(torch.zeros((2, 5), dtype=torch.float32, device="cuda"), ["i", "k"])
+    with pytest.raises(muscle.ArgumentError):
+        muscle.binary_einsum_(c, a, b)  # eltype is not the promotion
+
+
+# ------------------------------------------------------------ full-size config 2
+@pytest.mark.slow
+@pytest.mark.parametrize("T", [np.complex128, np.complex64])
+def test_config2_full_size(T):
+    from muscle.workloads import config2_inputs
+    A, ia, B, ib, ic = config2_inputs(16, seed=1, dtype=T)
+    a, b = dev_tensor(A, ia), dev_tensor(B, ib)
+    c = muscle.binary_einsum(a, b, out=ic)
+    C = host(c)
+    # sampled entries against the oracle's dot products
+    Am = np.reshape(np.transpose(A, (0, 2, 4, 1, 3, 5)), (4096, 4096), order="F").astype(np.complex128)
+    Bm = np.reshape(np.transpose(B, (0, 2, 4, 1, 3, 5)), (4096, 4096), order="F").astype(np.complex128)
+    Cm = np.reshape(np.transpose(C, (0, 2, 4, 1, 3, 5)), (4096, 4096), order="F")
+    rng = np.random.default_rng(0)
+    ms, ns = rng.integers(0, 4096, 64), rng.integers(0, 4096, 64)
+    ref = np.array([Am[m, :] @ Bm[:, n] for m, n in zip(ms, ns)])
+    got = Cm[ms, ns]
+    assert rel_err(got, ref) <= TOL[T]
+    # size-independent identity: sum_n C[m, n] = A[m, :] . (sum_n B[:, n])
+    lhs = Cm.astype(np.complex128).sum(axis=1)
+    rhs = Am @ Bm.sum(axis=1)
+    assert rel_err(lhs, rhs) <= (1e-10 if T == np.complex128 else 1e-4)

@@ -1,0 +1,166 @@
+"""Host-side logic that needs no GPU: the C ABI loads and exports what
+include/*.h declares, index classification / lowering (muscle_hip_describe),
+the Python mirror's data model and dispatch."""
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import muscle
+from muscle import _lib
+from muscle.describe import describe
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        names |= set(re.findall(r"MUSCLE_HIP_API\s+[\w\s\*]+?\b(muscle_hip_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(_lib.lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings():
+    assert _lib.lib.muscle_hip_status_string(0) == b"MUSCLE_HIP_OK"
+    assert _lib.lib.muscle_hip_status_string(2) == b"MUSCLE_HIP_DIM_MISMATCH"
+    assert _lib.lib.muscle_hip_version() >= 100
+
+
+# ---------------------------------------------------------------- lowering
+def test_config1_lowering():
+    d = describe(["i", "l"], [64, 96], ["i", "j", "k"], [64, 32, 48], ["k", "l", "j"], [48, 96, 32])
+    assert d["classes"] == {"i": "left", "l": "right", "j": "contract", "k": "contract"}
+    # C is column-major (i fastest): the lowering exchanges A and B so C's fastest mode is an N mode
+    assert d["swapped"] == 1 and (d["m"], d["n"], d["k"], d["batch"]) == (96, 64, 1536, 1)
+    assert d["kernel_name"] == "gemm_mfma"
+    assert d["flops"] == 8 * 64 * 96 * 1536
+    assert d["bytes"] == 16 * (64 * 32 * 48 + 48 * 96 * 32 + 64 * 96)
+
+
+def test_config2_lowering_interleaved():
+    a = ["a1", "c1", "a3", "c2", "a5", "c3"]
+    b = ["c1", "b1", "c2", "b2", "c3", "b3"]
+    c = ["a1", "b1", "a3", "b2", "a5", "b3"]
+    d = describe(c, [16] * 6, a, [16] * 6, b, [16] * 6)
+    assert (d["m"], d["n"], d["k"]) == (4096, 4096, 4096)
+    assert all(d["classes"][x] == "contract" for x in ["c1", "c2", "c3"])
+
+
+def test_fusion_of_adjacent_modes():
+    # A(i,j,k,l) col-major, B(k,l,m): (k,l) fuse to one K mode of 20, (i,j) to one M mode of 6
+    d = describe(["i", "j", "m"], [2, 3, 7], ["i", "j", "k", "l"], [2, 3, 4, 5], ["k", "l", "m"], [4, 5, 7])
+    assert (d["m"], d["n"], d["k"]) in [(6, 7, 20), (7, 6, 20)]
+
+
+def test_hyperindex_classification():
+    d = describe(["i", "k", "l"], [2, 4, 5], ["i", "j", "k"], [2, 3, 4], ["k", "l", "j"], [4, 5, 3])
+    assert d["classes"] == {"i": "left", "k": "batch", "l": "right", "j": "contract"}
+    assert d["batch"] == 4 and d["k"] == 3
+
+
+def test_single_operand_sum_and_diagonal():
+    # j appears only in A and not in C: summed over A alone
+    d = describe(["i", "k"], [2, 4], ["i", "j"], [2, 3], ["k"], [4])
+    assert d["classes"]["j"] == "sum_a"
+    # i repeated in A: its diagonal
+    d = describe(["i", "k"], [3, 4], ["i", "i"], [3, 3], ["k"], [4])
+    assert d["classes"]["i"] == "left" and d["m"] * d["n"] == 12
+
+
+def test_scalar_and_inner_product_lowering():
+    d = describe([], [], ["i", "j"], [3, 4], ["j", "i"], [4, 3])
+    assert (d["m"], d["n"], d["k"], d["batch"]) == (1, 1, 12, 1)
+    assert d["kernel_name"] == "reduce"
+    d = describe(["i", "j"], [2, 3], ["i", "j"], [2, 3], [], [])
+    assert d["k"] == 1 and d["m"] * d["n"] == 6
+
+
+@pytest.mark.parametrize("bad, exc", [
+    ((["i", "i"], [2, 2], ["i", "j"], [2, 3], ["j", "i"], [3, 2]), muscle.ArgumentError),  # repeated out
+    ((["x"], [2], ["i", "j"], [2, 3], ["j", "i"], [3, 2]), muscle.ArgumentError),  # out not in inputs
+    ((["i"], [2], ["i", "j"], [2, 3], ["j", "i"], [4, 2]), muscle.DimensionMismatch),  # extents differ
+])
+def test_lowering_errors(bad, exc):
+    with pytest.raises(exc):
+        describe(*bad)
+
+
+def test_promotion_rule_enforced():
+    with pytest.raises(muscle.ArgumentError):
+        describe(["i", "k"], [2, 4], ["i", "j"], [2, 3], ["j", "k"], [3, 4], dtypes=("f64", "c64", "f64"))
+    d = describe(["i", "k"], [2, 4], ["i", "j"], [2, 3], ["j", "k"], [3, 4], dtypes=("c64", "f64", "c64"))
+    assert d["flops"] == 4 * 2 * 4 * 3
+
+
+def test_strided_view_lowering():
+    # A is a view with non-unit strides: (i stride 2, j stride 20)
+    d = describe(["i", "k"], [5, 6], ["i", "j"], [5, 7], ["j", "k"], [7, 6],
+                 strides=(None, [2, 20], None))
+    assert (d["m"] * d["n"], d["k"]) == (30, 7)
+
+
+# ---------------------------------------------------------------- data model + dispatch
+def test_tensor_constructor_checks():
+    with pytest.raises(muscle.DimensionMismatch):
+        muscle.Tensor(torch.ones(2, 3), ["i"])
+    with pytest.raises(muscle.DimensionMismatch):
+        muscle.Tensor(torch.ones(2, 3), ["i", "i"])
+    t = muscle.Tensor(torch.ones(2, 2), ["i", "i"])
+    assert t.inds() == [muscle.Index("i"), muscle.Index("i")]
+    assert muscle.Index("i") == muscle.Index(muscle.Index("i"))
+    assert repr(muscle.Index("i")) == "index<i>"
+
+
+def test_host_arrays_dispatch_to_reference_backends():
+    a = muscle.Tensor(torch.ones(2, 3, dtype=torch.float64), ["i", "j"])
+    b = muscle.Tensor(torch.ones(3, 4, dtype=torch.float64), ["j", "k"])
+    assert muscle.choose_backend("binary_einsum", a.parent(), b.parent()) == muscle.BackendBase()
+    assert muscle.choose_backend("unary_einsum", a.parent()) == muscle.BackendOMEinsum()
+    # not part of this library: the reference's own "not implemented or not loaded" error
+    with pytest.raises(muscle.ArgumentError, match="not implemented or not loaded"):
+        muscle.binary_einsum(a, b)
+    with pytest.raises(muscle.ArgumentError, match="not implemented or not loaded"):
+        muscle.unary_einsum(a, dims=["i"])
+
+
+def test_with_backend_override_is_scoped():
+    a = muscle.Tensor(torch.ones(2), ["i"])
+    with muscle.with_backend(muscle.BackendHIP(precision="ozaki_int8")) as be:
+        assert muscle.choose_backend("binary_einsum", a.parent(), a.parent()) is be
+        with muscle.with_backend(muscle.BackendBase()):
+            assert muscle.choose_backend("hadamard", a.parent(), a.parent()) == muscle.BackendBase()
+        assert muscle.choose_backend("hadamard", a.parent(), a.parent()) is be
+    assert muscle.choose_backend("binary_einsum", a.parent(), a.parent()) == muscle.BackendBase()
+
+
+def test_device_rules_select_hip():
+    assert muscle.choose_backend_rule("binary_einsum", muscle.DomainROCm(), muscle.DomainROCm()) == muscle.BackendHIP()
+    assert muscle.choose_backend_rule("hadamard!", muscle.DomainROCm(), muscle.DomainROCm(),
+                                      muscle.DomainROCm()) == muscle.BackendHIP()
+    assert muscle.choose_backend_rule("unary_einsum", muscle.DomainROCm()) == muscle.BackendHIP()
+
+
+def test_hip_backend_rejects_host_arrays():
+    a = muscle.Tensor(torch.ones(2, 3, dtype=torch.float64), ["i", "j"])
+    with muscle.with_backend(muscle.BackendHIP()):
+        with pytest.raises(muscle.ArgumentError, match="device"):
+            muscle.binary_einsum(a, a)
+
+
+def test_isapprox_is_permutation_aware():
+    x = torch.arange(6, dtype=torch.float64).reshape(2, 3)
+    a = muscle.Tensor(x, ["i", "j"])
+    b = muscle.Tensor(x.T.contiguous(), ["j", "i"])
+    assert a.isapprox(b)
+    assert not a.isapprox(muscle.Tensor(x.T.contiguous() + 1, ["j", "i"]))
