@@ -15,10 +15,12 @@
 // MFMA operand fetch is one ds_read of a T.  The next K-tile is fetched from
 // global memory into registers while the current one is multiplied.
 //
-// v_mfma_{f64_16x16x4f64, f32_16x16x4f32} lane layout (64 lanes):
+// v_mfma_{f64_16x16x4f64, f32_16x16x4f32} lane layout on gfx950 (64 lanes),
+// established with exact small-integer operands on the device:
 //   A operand: lane l holds A[i = l % 16][k = l / 16]
 //   B operand: lane l holds B[k = l / 16][j = l % 16]
-//   D (4 regs): lane l, reg r holds D[i = 4 * (l / 16) + r][j = l % 16]
+//   D (4 regs), f32: lane l, reg r holds D[i = 4 * (l / 16) + r][j = l % 16]
+//   D (4 regs), f64: lane l, reg r holds D[i = l / 16 + 4 * r][j = l % 16]
 #include "device_types.hpp"
 #include "kernels.hpp"
 
@@ -33,6 +35,8 @@ struct Mfma<double> {
   static __device__ inline acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // row of D held by (lane, reg)
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -40,6 +44,7 @@ struct Mfma<float> {
   static __device__ inline acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
@@ -175,7 +180,7 @@ __global__ __launch_bounds__(NT) void gemm_tile_kernel(GemmArgs p) {
       const int64_t cn = offC0 + p.colC[n];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + r;
+        const int m = m0 + wm * 32 + i * 16 + M_::row(lane, r);
         if (m < p.M) store_elem<T, CC>(p.C, cn + p.rowC[m], Cplx<T>{accr[i][j][r], acci[i][j][r]});
       }
     }
