@@ -123,6 +123,15 @@ MUSCLE_HIP_API const char* muscle_hip_status_string(int status);
 /* Detail of the last failing call on this thread. */
 MUSCLE_HIP_API const char* muscle_hip_last_error(void);
 MUSCLE_HIP_API int muscle_hip_version(void);
+/* Handle options (muscle_hip_set_option). */
+typedef enum {
+  /* 3: complex x complex contractions use three real MFMA products per
+   *    complex product (Gauss / 3M) -- the default;
+   * 4: four real products (4M). */
+  MUSCLE_HIP_OPTION_COMPLEX_MULT = 0
+} muscle_hip_option_t;
+MUSCLE_HIP_API int muscle_hip_set_option(muscle_hip_handle_t handle, int option, int64_t value);
+
 /* Number of cached contraction plans, and drop them all. */
 MUSCLE_HIP_API int64_t muscle_hip_plan_cache_size(muscle_hip_handle_t handle);
 MUSCLE_HIP_API int muscle_hip_plan_cache_clear(muscle_hip_handle_t handle);

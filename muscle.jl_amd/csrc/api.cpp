@@ -283,6 +283,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     g.K = (int32_t)d.K;
     g.a_kmajor = p.a_kmajor;
     g.b_kmajor = p.b_kmajor;
+    g.complex3m = h.complex_mult == 3;
     // grid.y carries the batch; split launches beyond the 65535 limit
     for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
       g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);
@@ -406,6 +407,21 @@ int muscle_hip_set_stream(muscle_hip_handle_t handle, void* stream) {
     if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
     std::lock_guard<std::mutex> lk(h->mu);
     h->stream = reinterpret_cast<hipStream_t>(stream);
+  });
+}
+
+int muscle_hip_set_option(muscle_hip_handle_t handle, int option, int64_t value) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    switch (option) {
+      case MUSCLE_HIP_OPTION_COMPLEX_MULT:
+        if (value != 3 && value != 4) fail(MUSCLE_HIP_INVALID_ARGUMENT, "complex multiplication must be 3 or 4");
+        h->complex_mult = (int)value;
+        break;
+      default: fail(MUSCLE_HIP_INVALID_ARGUMENT, "unknown option");
+    }
   });
 }
 

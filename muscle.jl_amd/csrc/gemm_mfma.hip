@@ -6,14 +6,23 @@
 // permutedims(c) sequence (src/Operations/binary_einsum.jl:89-95) with one
 // kernel: operand tiles are gathered from their strided layout straight into
 // LDS through per-mode offset tables, multiplied on MFMA, and scattered to C's
-// own layout in the epilogue.  Complex products use four real MFMAs (4M):
-//   re += ar*br - ai*bi,  im += ar*bi + ai*br.
+// own layout in the epilogue.
 //
-// Tile: 64 (M) x 64 (N) x 16 (K) per 256-thread workgroup, four waves in a
+// Complex products run on real MFMAs either as 4M
+//   re += ar*br - ai*bi,  im += ar*bi + ai*br               (4 MFMAs)
+// or as 3M (Gauss):
+//   P1 += ar*br, P2 += ai*bi, P3 += (ar+ai)*(br+bi);
+//   re = P1 - P2, im = P3 - P1 - P2                         (3 MFMAs)
+//
+// Tile: 64 (M) x 64 (N) x BK (K) per 256-thread workgroup, four waves in a
 // 2 x 2 arrangement, each wave a 32 x 32 block of 2 x 2 MFMA 16x16x4 tiles.
-// Operands are staged in LDS as separate real and imaginary planes so each
-// MFMA operand fetch is one ds_read of a T.  The next K-tile is fetched from
-// global memory into registers while the current one is multiplied.
+// Operands are staged in LDS as separate real and imaginary planes, double
+// buffered, so one barrier per K-tile suffices.  The global gather of K-tile
+// t+1 and the offset-table reads of K-tile t+2 are in flight while K-tile t
+// is multiplied.  Rows of 64 + 16 elements put the four 16-lane groups of an
+// MFMA operand read in disjoint LDS banks.  Workgroups are renumbered so each
+// XCD (the dispatcher deals workgroups round-robin over 8 XCDs) gets a
+// contiguous, grouped range of tiles and reuses A and B panels in its own L2.
 //
 // v_mfma_{f64_16x16x4f64, f32_16x16x4f32} lane layout on gfx950 (64 lanes),
 // established with exact small-integer operands on the device:
@@ -35,7 +44,6 @@ struct Mfma<double> {
   static __device__ inline acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
-  // row of D held by (lane, reg)
   static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
@@ -47,99 +55,132 @@ struct Mfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
-constexpr int LDP = 4;  // LDS row padding (elements)
+constexpr int BM = 64, BN = 64, NT = 256;
+constexpr int LDP = 16;  // LDS row padding (elements)
+constexpr int GROUP_M = 8;
 
-template <typename T, bool CA, bool CB>
-__global__ __launch_bounds__(NT) void gemm_tile_kernel(GemmArgs p) {
+}  // namespace
+
+// Tile order: XCD-contiguous ranges, then groups of GROUP_M tile rows walked
+// row-fastest so concurrently running workgroups share A and B panels.
+__device__ void gemm_tile_coords(int L, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
+  if ((nwg & 7) == 0) L = (L & 7) * (nwg >> 3) + (L >> 3);
+  const int per_group = GROUP_M * tiles_n;
+  const int g = L / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(tiles_m - first_m, GROUP_M);
+  const int r = L - g * per_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+namespace {
+
+template <typename T, bool CA, bool CB, bool G3, int BK>
+__global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   using M_ = Mfma<T>;
   using acc_t = typename M_::acc_t;
-  __shared__ T As[CA ? 2 : 1][BK][BM + LDP];
-  __shared__ T Bs[CB ? 2 : 1][BK][BN + LDP];
+  constexpr int NE = BK / 4;  // elements of A (and of B) each thread gathers per K-tile
+  constexpr int PA = CA ? 2 : 1, PB = CB ? 2 : 1;
+  constexpr bool CC = CA || CB;
+  static_assert(!G3 || (CA && CB), "3M needs two complex operands");
+  __shared__ T As[2][PA][BK][BM + LDP];
+  __shared__ T Bs[2][PB][BK][BN + LDP];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-
-  // tile coordinates; consecutive workgroups walk N first so neighbours share A rows
-  const int tile = blockIdx.x;
-  const int tm = tile / p.tiles_n;
-  const int tn = tile - tm * p.tiles_n;
+  int tm, tn;
+  gemm_tile_coords(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int bz = blockIdx.y;
   const int64_t offA0 = p.bA[bz], offB0 = p.bB[bz], offC0 = p.bC[bz];
 
-  // global -> register staging: 4 elements of A and 4 of B per thread
-  int a_mm[4], a_kk[4], b_nn[4], b_kk[4];
+  // which (m|n, k) of the tile each thread gathers
+  int a_mm[NE], a_kk[NE], b_nn[NE], b_kk[NE];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < NE; ++i) {
     const int e = tid + NT * i;
     if (p.a_kmajor) {
-      a_kk[i] = e & (BK - 1);
-      a_mm[i] = e >> 4;
+      a_kk[i] = e % BK;
+      a_mm[i] = e / BK;
     } else {
       a_mm[i] = e & (BM - 1);
       a_kk[i] = e >> 6;
     }
     if (p.b_kmajor) {
-      b_kk[i] = e & (BK - 1);
-      b_nn[i] = e >> 4;
+      b_kk[i] = e % BK;
+      b_nn[i] = e / BK;
     } else {
       b_nn[i] = e & (BN - 1);
       b_kk[i] = e >> 6;
     }
   }
-  int64_t rowA_r[4], colB_r[4];
-  bool a_ok[4], b_ok[4];
+  int64_t rowA_r[NE], colB_r[NE];
+  bool a_ok[NE], b_ok[NE];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < NE; ++i) {
     const int m = m0 + a_mm[i], n = n0 + b_nn[i];
     a_ok[i] = m < p.M;
     b_ok[i] = n < p.N;
-    rowA_r[i] = a_ok[i] ? p.rowA[m] : 0;
-    colB_r[i] = b_ok[i] ? p.colB[n] : 0;
+    rowA_r[i] = p.rowA[a_ok[i] ? m : p.M - 1] + offA0;
+    colB_r[i] = p.colB[b_ok[i] ? n : p.N - 1] + offB0;
   }
+  const int nk = (p.K + BK - 1) / BK;
 
-  Cplx<T> ra[4], rb[4];
-  auto fetch = [&](int k0) {
+  // K-offset table entries of the tile gathered next (indices clamped in range)
+  int64_t tA[NE], tB[NE];
+  auto load_tables = [&](int kt) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int ka = k0 + a_kk[i];
-      ra[i] = Cplx<T>{T(0), T(0)};
-      if (a_ok[i] && ka < p.K) ra[i] = load_elem<T, CA>(p.A, offA0 + rowA_r[i] + p.kA[ka]);
-      const int kb = k0 + b_kk[i];
-      rb[i] = Cplx<T>{T(0), T(0)};
-      if (b_ok[i] && kb < p.K) rb[i] = load_elem<T, CB>(p.B, offB0 + colB_r[i] + p.kB[kb]);
+    for (int i = 0; i < NE; ++i) {
+      tA[i] = p.kA[min(kt * BK + a_kk[i], p.K - 1)];
+      tB[i] = p.kB[min(kt * BK + b_kk[i], p.K - 1)];
     }
   };
-  auto stash = [&]() {
+  Cplx<T> ra[NE], rb[NE];
+  auto gather = [&](int kt) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      As[0][a_kk[i]][a_mm[i]] = ra[i].re;
-      if constexpr (CA) As[CA ? 1 : 0][a_kk[i]][a_mm[i]] = ra[i].im;
-      Bs[0][b_kk[i]][b_nn[i]] = rb[i].re;
-      if constexpr (CB) Bs[CB ? 1 : 0][b_kk[i]][b_nn[i]] = rb[i].im;
+    for (int i = 0; i < NE; ++i) {
+      ra[i] = load_elem<T, CA>(p.A, rowA_r[i] + tA[i]);
+      rb[i] = load_elem<T, CB>(p.B, colB_r[i] + tB[i]);
+      const bool ka = kt * BK + a_kk[i] < p.K, kb = kt * BK + b_kk[i] < p.K;
+      if (!(a_ok[i] && ka)) ra[i] = Cplx<T>{T(0), T(0)};
+      if (!(b_ok[i] && kb)) rb[i] = Cplx<T>{T(0), T(0)};
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      As[buf][0][a_kk[i]][a_mm[i]] = ra[i].re;
+      if constexpr (CA) As[buf][PA - 1][a_kk[i]][a_mm[i]] = ra[i].im;
+      Bs[buf][0][b_kk[i]][b_nn[i]] = rb[i].re;
+      if constexpr (CB) Bs[buf][PB - 1][b_kk[i]][b_nn[i]] = rb[i].im;
     }
   };
 
-  acc_t accr[2][2], acci[2][2];
+  acc_t acc0[2][2], acc1[2][2], acc2[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        accr[i][j][r] = T(0);
-        acci[i][j][r] = T(0);
+        acc0[i][j][r] = T(0);
+        acc1[i][j][r] = T(0);
+        acc2[i][j][r] = T(0);
       }
 
-  const int nk = (p.K + BK - 1) / BK;
-  fetch(0);
-  stash();
+  load_tables(0);
+  gather(0);
+  stash(0);
+  if (nk > 1) load_tables(1);
   __syncthreads();
+
   for (int kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) fetch((kt + 1) * BK);
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gather(kt + 1);
+    if (kt + 2 < nk) load_tables(kt + 2);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
@@ -147,30 +188,44 @@ __global__ __launch_bounds__(NT) void gemm_tile_kernel(GemmArgs p) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int mm = wm * 32 + i * 16 + (lane & 15);
-        ar[i] = As[0][kr][mm];
-        ai[i] = CA ? -As[CA ? 1 : 0][kr][mm] : T(0);  // negated: feeds re -= ai * bi
+        ar[i] = As[cur][0][kr][mm];
+        ai[i] = CA ? As[cur][PA - 1][kr][mm] : T(0);
         const int nn = wn * 32 + i * 16 + (lane & 15);
-        br[i] = Bs[0][kr][nn];
-        bi[i] = CB ? Bs[CB ? 1 : 0][kr][nn] : T(0);
+        br[i] = Bs[cur][0][kr][nn];
+        bi[i] = CB ? Bs[cur][PB - 1][kr][nn] : T(0);
       }
+      if constexpr (G3) {
+        T as[2], bs[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          accr[i][j] = M_::mma(ar[i], br[j], accr[i][j]);
-          if constexpr (CA && CB) accr[i][j] = M_::mma(ai[i], bi[j], accr[i][j]);
-          if constexpr (CB) acci[i][j] = M_::mma(ar[i], bi[j], acci[i][j]);
-          if constexpr (CA) acci[i][j] = M_::mma(-ai[i], br[j], acci[i][j]);
+        for (int i = 0; i < 2; ++i) {
+          as[i] = ar[i] + ai[i];
+          bs[i] = br[i] + bi[i];
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc0[i][j] = M_::mma(ar[i], br[j], acc0[i][j]);
+            acc1[i][j] = M_::mma(ai[i], bi[j], acc1[i][j]);
+            acc2[i][j] = M_::mma(as[i], bs[j], acc2[i][j]);
+          }
+      } else {
+        // acc0: real part, acc1: imaginary part
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc0[i][j] = M_::mma(ar[i], br[j], acc0[i][j]);
+            if constexpr (CA && CB) acc0[i][j] = M_::mma(-ai[i], bi[j], acc0[i][j]);
+            if constexpr (CB) acc1[i][j] = M_::mma(ar[i], bi[j], acc1[i][j]);
+            if constexpr (CA) acc1[i][j] = M_::mma(ai[i], br[j], acc1[i][j]);
+          }
+      }
     }
+    if (kt + 1 < nk) stash(cur ^ 1);
     __syncthreads();
-    if (kt + 1 < nk) {
-      stash();
-      __syncthreads();
-    }
   }
 
-  constexpr bool CC = CA || CB;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -181,15 +236,24 @@ __global__ __launch_bounds__(NT) void gemm_tile_kernel(GemmArgs p) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * 32 + i * 16 + M_::row(lane, r);
-        if (m < p.M) store_elem<T, CC>(p.C, cn + p.rowC[m], Cplx<T>{accr[i][j][r], acci[i][j][r]});
+        if (m >= p.M) continue;
+        T re, im;
+        if constexpr (G3) {
+          re = acc0[i][j][r] - acc1[i][j][r];
+          im = acc2[i][j][r] - acc0[i][j][r] - acc1[i][j][r];
+        } else {
+          re = acc0[i][j][r];
+          im = acc1[i][j][r];
+        }
+        store_elem<T, CC>(p.C, cn + p.rowC[m], Cplx<T>{re, im});
       }
     }
 }
 
-template <typename T, bool CA, bool CB>
+template <typename T, bool CA, bool CB, bool G3>
 void launch_t(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.batch);
-  hipLaunchKernelGGL((gemm_tile_kernel<T, CA, CB>), grid, dim3(NT), 0, s, g);
+  hipLaunchKernelGGL((gemm_tile_kernel<T, CA, CB, G3, 16>), grid, dim3(NT), 0, s, g);
 }
 
 }  // namespace
@@ -198,16 +262,19 @@ void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g0, hipStream_
   GemmArgs g = g0;
   g.tiles_m = (g.M + BM - 1) / BM;
   g.tiles_n = (g.N + BN - 1) / BN;
-  const int key = (dbl ? 4 : 0) | (ca ? 2 : 0) | (cb ? 1 : 0);
+  const bool g3 = ca && cb && g.complex3m;
+  const int key = (dbl ? 8 : 0) | (ca ? 4 : 0) | (cb ? 2 : 0) | (g3 ? 1 : 0);
   switch (key) {
-    case 0: launch_t<float, false, false>(g, s); break;
-    case 1: launch_t<float, false, true>(g, s); break;
-    case 2: launch_t<float, true, false>(g, s); break;
-    case 3: launch_t<float, true, true>(g, s); break;
-    case 4: launch_t<double, false, false>(g, s); break;
-    case 5: launch_t<double, false, true>(g, s); break;
-    case 6: launch_t<double, true, false>(g, s); break;
-    case 7: launch_t<double, true, true>(g, s); break;
+    case 0: launch_t<float, false, false, false>(g, s); break;
+    case 2: launch_t<float, false, true, false>(g, s); break;
+    case 4: launch_t<float, true, false, false>(g, s); break;
+    case 6: launch_t<float, true, true, false>(g, s); break;
+    case 7: launch_t<float, true, true, true>(g, s); break;
+    case 8: launch_t<double, false, false, false>(g, s); break;
+    case 10: launch_t<double, false, true, false>(g, s); break;
+    case 12: launch_t<double, true, false, false>(g, s); break;
+    case 14: launch_t<double, true, true, false>(g, s); break;
+    case 15: launch_t<double, true, true, true>(g, s); break;
   }
 }
 

@@ -45,6 +45,7 @@ struct Handle {
   std::list<std::string> lru;
   void* ws = nullptr;
   size_t ws_size = 0;
+  int complex_mult = 3;  // MUSCLE_HIP_OPTION_COMPLEX_MULT
 
   ~Handle();
   void ensure_workspace(size_t bytes);

@@ -48,6 +48,7 @@ struct GemmArgs {
   int32_t a_kmajor;  // A's contiguous direction is K (else M)
   int32_t b_kmajor;  // B's contiguous direction is K (else N)
   int32_t tiles_m, tiles_n;
+  int32_t complex3m;  // complex x complex with 3 real products (Gauss) instead of 4
 };
 // dbl: FP64 (else FP32) arithmetic; ca/cb: operand is complex.
 void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g, hipStream_t s);
