@@ -81,6 +81,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2_c64", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="native")
+    ap.add_argument("--complex-mult", type=int, default=3, choices=[3, 4])
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -100,7 +101,7 @@ def main() -> int:
     A, ia, B, ib, kw = make_inputs(args.workload)
     a = muscle.Tensor(torch.from_numpy(np.asfortranarray(A)).to(dev), ia)
     b = muscle.Tensor(torch.from_numpy(np.asfortranarray(B)).to(dev), ib)
-    backend = muscle.BackendHIP(precision=args.precision)
+    backend = muscle.BackendHIP(precision=args.precision, complex_mult=args.complex_mult)
 
     def step():
         with muscle.with_backend(backend):
@@ -157,6 +158,7 @@ def main() -> int:
         if args.precision == "native" else f"complex128 via {args.precision}",
         "data": "synthetic (seeded numpy CN(0,1)/sqrt(K))",
         "config": {"workload": WORKLOADS[args.workload][0], "precision": args.precision,
+                   "complex_mult": args.complex_mult,
                    "m": info["m"], "n": info["n"], "k": info["k"], "batch": info["batch"],
                    "kernel": info["kernel"], "flops_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",

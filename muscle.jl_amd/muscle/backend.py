@@ -64,17 +64,20 @@ class BackendDagger(Backend):
 class BackendHIP(Backend):
     """The MI355X backend.  ``precision`` selects the arithmetic of binary_einsum
     ("native", "fp32", "bf16x3", "fp16x2", "ozaki_int8"); ``target_rel_error``
-    sets the Ozaki slice count.  Like the keyword arguments BackendCuTENSOR
+    sets the Ozaki slice count; ``complex_mult`` picks 3M or 4M complex
+    products.  Like the keyword arguments BackendCuTENSOR
     forwards to cuTENSOR (ext/MuscleCUDAExt.jl:22,37), they only tune the
     library call."""
 
     precision: str = "native"
     target_rel_error: float = 1e-12
+    complex_mult: int = 3  # complex x complex: 3 real products (Gauss) or 4
 
     def __repr__(self):
-        if self.precision == "native":
+        if self.precision == "native" and self.complex_mult == 3:
             return "BackendHIP()"
-        return f"BackendHIP(precision={self.precision!r}, target_rel_error={self.target_rel_error:g})"
+        return (f"BackendHIP(precision={self.precision!r}, target_rel_error={self.target_rel_error:g}, "
+                f"complex_mult={self.complex_mult})")
 
 
 # ---------------------------------------------------------------- Domains

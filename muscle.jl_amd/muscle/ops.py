@@ -115,6 +115,7 @@ def hip_binary_einsum_(backend: BackendHIP, c: Tensor, a: Tensor, b: Tensor) -> 
     prec = _lib.PRECISION.get(backend.precision)
     if prec is None:
         raise ArgumentError(f"unknown precision {backend.precision!r}")
+    _lib.check(_lib.lib.muscle_hip_set_option(h, 0, int(backend.complex_mult)))
     _lib.check(_lib.lib.muscle_hip_binary_einsum(h, dc.ref, da.ref, db.ref, prec, float(backend.target_rel_error),
                                                  ctypes.byref(info)))
     _record(info)

@@ -84,6 +84,19 @@ def test_binary_parity(case, T):
     assert rel_err(C, ref) <= TOL[T], (name, rel_err(C, ref))
 
 
+@pytest.mark.parametrize("mult", [3, 4])
+@pytest.mark.parametrize("T", [np.complex128, np.complex64])
+def test_complex_3m_and_4m(mult, T):
+    rng = np.random.default_rng(21)
+    A, B = randn(rng, (130, 70, 3), T), randn(rng, (70, 3, 150), T)
+    ref = bb.binary_einsum_base(["l", "i"], A.astype(np.complex128), ["i", "j", "k"], B.astype(np.complex128),
+                                ["j", "k", "l"])
+    with muscle.with_backend(muscle.BackendHIP(complex_mult=mult)):
+        c = muscle.binary_einsum(dev_tensor(A, ["i", "j", "k"]), dev_tensor(B, ["j", "k", "l"]), out=["l", "i"])
+    assert muscle.last_info()["kernel"] == 1
+    assert rel_err(host(c), ref) <= TOL[T]
+
+
 def test_config1_golden():
     """Config 1 against the committed golden output (tests/golden/config1_c64.npz)."""
     import os
