@@ -159,6 +159,23 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   } else {
     p->kind = Plan::REDUCE;
   }
+  // split-K for tile GEMMs with too few output tiles to fill 256 CUs:
+  // slices of at least 64 along K, partial sums bounded to 256 MiB
+  p->gemm_ksplit = 1;
+  p->gemm_kchunk = (int)std::min<int64_t>(d.K, INT32_MAX);
+  if (p->kind == Plan::GEMM) {
+    const int64_t tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64) * d.B;
+    if (tiles < 512 && d.K >= 128) {
+      int64_t ks = std::min<int64_t>((1024 + tiles - 1) / tiles, d.K / 64);
+      const int64_t elem = (int64_t)dtype_size(d.dt[2]);
+      const int64_t cap = (int64_t(256) << 20) / std::max<int64_t>(1, d.M * d.N * d.B * elem);
+      ks = std::max<int64_t>(1, std::min<int64_t>({ks, 64, cap}));
+      int64_t kc = (d.K + ks - 1) / ks;
+      kc = (kc + 15) / 16 * 16;
+      p->gemm_kchunk = (int)kc;
+      p->gemm_ksplit = (int)((d.K + kc - 1) / kc);
+    }
+  }
   // reduction configuration (also the fallback of the other kinds)
   const int64_t outputs = d.M * d.N * d.B;
   int gshift = 0;
@@ -252,6 +269,8 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   const size_t csize = dtype_size(d.dt[2]);
   size_t partial_off = need;
   if (p.kind == Plan::REDUCE && p.nsplit > 1) need += align_up((size_t)(d.M * d.N * d.B) * csize * p.nsplit);
+  if (p.kind == Plan::GEMM && p.gemm_ksplit > 1)
+    need += align_up((size_t)(d.M * d.N * d.B) * csize * p.gemm_ksplit);
   size_t split_off = need;
   if (p.kind == Plan::SPLIT) need += split_workspace_bytes(p);
   if (need) h.ensure_workspace(need);
@@ -284,6 +303,9 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     g.a_kmajor = p.a_kmajor;
     g.b_kmajor = p.b_kmajor;
     g.complex3m = h.complex_mult == 3;
+    g.ksplit = p.gemm_ksplit;
+    g.kchunk = p.gemm_kchunk;
+    g.partial = (char*)h.ws + partial_off;
     // grid.y carries the batch; split launches beyond the 65535 limit
     for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
       g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);

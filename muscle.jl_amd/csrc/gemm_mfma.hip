@@ -96,6 +96,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   const int m0 = tm * BM, n0 = tn * BN;
   const int bz = blockIdx.y;
   const int64_t offA0 = p.bA[bz], offB0 = p.bB[bz], offC0 = p.bC[bz];
+  // this workgroup's K range (split-K slice)
+  const int kbeg = blockIdx.z * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
 
   // which (m|n, k) of the tile each thread gathers
   int a_mm[NE], a_kk[NE], b_nn[NE], b_kk[NE];
@@ -127,15 +130,15 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
     rowA_r[i] = p.rowA[a_ok[i] ? m : p.M - 1] + offA0;
     colB_r[i] = p.colB[b_ok[i] ? n : p.N - 1] + offB0;
   }
-  const int nk = (p.K + BK - 1) / BK;
+  const int nk = (kend - kbeg + BK - 1) / BK;
 
   // K-offset table entries of the tile gathered next (indices clamped in range)
   int64_t tA[NE], tB[NE];
   auto load_tables = [&](int kt) {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
-      tA[i] = p.kA[min(kt * BK + a_kk[i], p.K - 1)];
-      tB[i] = p.kB[min(kt * BK + b_kk[i], p.K - 1)];
+      tA[i] = p.kA[min(kbeg + kt * BK + a_kk[i], kend - 1)];
+      tB[i] = p.kB[min(kbeg + kt * BK + b_kk[i], kend - 1)];
     }
   };
   Cplx<T> ra[NE], rb[NE];
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
     for (int i = 0; i < NE; ++i) {
       ra[i] = load_elem<T, CA>(p.A, rowA_r[i] + tA[i]);
       rb[i] = load_elem<T, CB>(p.B, colB_r[i] + tB[i]);
-      const bool ka = kt * BK + a_kk[i] < p.K, kb = kt * BK + b_kk[i] < p.K;
+      const bool ka = kbeg + kt * BK + a_kk[i] < kend, kb = kbeg + kt * BK + b_kk[i] < kend;
       if (!(a_ok[i] && ka)) ra[i] = Cplx<T>{T(0), T(0)};
       if (!(b_ok[i] && kb)) rb[i] = Cplx<T>{T(0), T(0)};
     }
@@ -226,13 +229,16 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
     __syncthreads();
   }
 
+  // partial sums of a split-K slice go to the workspace, [split][batch][m][n]
+  const bool to_partial = p.ksplit > 1;
+  const int64_t pbase = ((int64_t)blockIdx.z * gridDim.y + bz) * (int64_t)p.M * p.N;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int n = n0 + wn * 32 + j * 16 + (lane & 15);
       if (n >= p.N) continue;
-      const int64_t cn = offC0 + p.colC[n];
+      const int64_t cn = to_partial ? pbase + n : offC0 + p.colC[n];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * 32 + i * 16 + M_::row(lane, r);
@@ -245,15 +251,40 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
           re = acc0[i][j][r];
           im = acc1[i][j][r];
         }
-        store_elem<T, CC>(p.C, cn + p.rowC[m], Cplx<T>{re, im});
+        if (to_partial) store_elem<T, CC>(p.partial, cn + (int64_t)m * p.N, Cplx<T>{re, im});
+        else store_elem<T, CC>(p.C, cn + p.rowC[m], Cplx<T>{re, im});
       }
     }
 }
 
+// C[bz, m, n] = sum over slices s of partial[s][bz][m][n], in slice order.
+template <typename T, bool CC>
+__global__ __launch_bounds__(NT) void splitk_finish_kernel(GemmArgs p) {
+  const int64_t mn = (int64_t)p.M * p.N;
+  const int64_t total = mn * p.batch;
+  const int64_t o = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (o >= total) return;
+  T re = 0, im = 0;
+  for (int s = 0; s < p.ksplit; ++s) {
+    const Cplx<T> v = load_elem<T, CC>(p.partial, (int64_t)s * total + o);
+    re += v.re;
+    im += v.im;
+  }
+  const int bz = (int)(o / mn);
+  const int64_t r = o - bz * mn;
+  const int m = (int)(r / p.N), n = (int)(r - (int64_t)m * p.N);
+  store_elem<T, CC>(p.C, p.bC[bz] + p.rowC[m] + p.colC[n], Cplx<T>{re, im});
+}
+
 template <typename T, bool CA, bool CB, bool G3>
 void launch_t(const GemmArgs& g, hipStream_t s) {
-  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.batch);
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.batch, (unsigned)g.ksplit);
   hipLaunchKernelGGL((gemm_tile_kernel<T, CA, CB, G3, 16>), grid, dim3(NT), 0, s, g);
+  if (g.ksplit > 1) {
+    const int64_t total = (int64_t)g.M * g.N * g.batch;
+    hipLaunchKernelGGL((splitk_finish_kernel<T, CA || CB>), dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0, s,
+                       g);
+  }
 }
 
 }  // namespace
@@ -262,6 +293,10 @@ void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g0, hipStream_
   GemmArgs g = g0;
   g.tiles_m = (g.M + BM - 1) / BM;
   g.tiles_n = (g.N + BN - 1) / BN;
+  if (g.ksplit < 1) {
+    g.ksplit = 1;
+    g.kchunk = g.K;
+  }
   const bool g3 = ca && cb && g.complex3m;
   const int key = (dbl ? 8 : 0) | (ca ? 4 : 0) | (cb ? 2 : 0) | (g3 ? 1 : 0);
   switch (key) {

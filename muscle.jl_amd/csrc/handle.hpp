@@ -25,6 +25,9 @@ struct Plan {
   std::vector<size_t> tab_at;
   int64_t* dtab = nullptr;
   bool a_kmajor = false, b_kmajor = false;
+  // tile GEMM split-K
+  int gemm_ksplit = 1;
+  int gemm_kchunk = 0;
   // reduction kernel configuration
   int gshift = 0;
   int nsplit = 1;

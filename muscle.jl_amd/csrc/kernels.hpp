@@ -49,6 +49,11 @@ struct GemmArgs {
   int32_t b_kmajor;  // B's contiguous direction is K (else N)
   int32_t tiles_m, tiles_n;
   int32_t complex3m;  // complex x complex with 3 real products (Gauss) instead of 4
+  // split-K: grid.z = ksplit slices of kchunk (a multiple of the K-tile); with
+  // ksplit > 1 every slice writes a partial [ksplit][batch][M][N] array in the
+  // compute type and a second kernel adds the slices in order into C.
+  int32_t ksplit, kchunk;
+  void* partial;
 };
 // dbl: FP64 (else FP32) arithmetic; ca/cb: operand is complex.
 void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g, hipStream_t s);

@@ -54,6 +54,8 @@ CASES = [
     ("config1 out^T", ["i", "j", "k"], (64, 32, 48), ["k", "l", "j"], (48, 96, 32), ["l", "i"]),
     ("ragged", ["i", "j"], (67, 45), ["j", "k"], (45, 93), ["i", "k"]),
     ("ragged2", ["i", "j"], (33, 17), ["k", "j"], (129, 17), ["k", "i"]),
+    ("split-K ragged", ["i", "j"], (70, 1001), ["j", "k"], (1001, 50), ["i", "k"]),
+    ("split-K batched", ["i", "j", "h"], (40, 777, 3), ["j", "k", "h"], (777, 33, 3), ["k", "h", "i"]),
     ("k-major A", ["j", "i"], (40, 70), ["j", "k"], (40, 50), ["i", "k"]),
     ("interleaved d=4", ["a1", "c1", "a3", "c2", "a5", "c3"], (4,) * 6, ["c1", "b1", "c2", "b2", "c3", "b3"], (4,) * 6,
      ["a1", "b1", "a3", "b2", "a5", "b3"]),
