@@ -82,6 +82,7 @@ def main() -> int:
     ap.add_argument("--workload", default="c2_c64", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="native")
     ap.add_argument("--complex-mult", type=int, default=3, choices=[3, 4])
+    ap.add_argument("--target", type=float, default=1e-10, help="target relative error of the Ozaki rung")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -101,7 +102,8 @@ def main() -> int:
     A, ia, B, ib, kw = make_inputs(args.workload)
     a = muscle.Tensor(torch.from_numpy(np.asfortranarray(A)).to(dev), ia)
     b = muscle.Tensor(torch.from_numpy(np.asfortranarray(B)).to(dev), ib)
-    backend = muscle.BackendHIP(precision=args.precision, complex_mult=args.complex_mult)
+    backend = muscle.BackendHIP(precision=args.precision, complex_mult=args.complex_mult,
+                                target_rel_error=args.target)
 
     def step():
         with muscle.with_backend(backend):

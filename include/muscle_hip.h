@@ -192,7 +192,7 @@ MUSCLE_HIP_API int muscle_hip_unary_einsum(muscle_hip_handle_t handle,
 MUSCLE_HIP_API int muscle_hip_describe(const muscle_hip_tensor_t* c,
                                        const muscle_hip_tensor_t* a,
                                        const muscle_hip_tensor_t* b,
-                                       int precision,
+                                       int precision, double target_rel_error,
                                        muscle_hip_info_t* info,
                                        int32_t* modes_out, int32_t* class_out,
                                        int32_t* nmodes_out);

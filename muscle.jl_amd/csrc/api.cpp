@@ -78,8 +78,7 @@ void Handle::ensure_workspace(size_t bytes) {
 
 Handle::~Handle() {
   if (ws) (void)hipFree(ws);
-  for (auto& kv : cache)
-    if (kv.second->dtab) (void)hipFree(kv.second->dtab);
+  cache.clear();  // plans free their device tables
 }
 
 Plan* Handle::find_plan(const std::string& key) {
@@ -97,10 +96,7 @@ Plan* Handle::insert_plan(const std::string& key, std::unique_ptr<Plan> p) {
     const std::string victim = lru.back();
     lru.pop_back();
     auto it = cache.find(victim);
-    if (it != cache.end()) {
-      if (it->second->dtab) (void)hipFree(it->second->dtab);
-      cache.erase(it);
-    }
+    if (it != cache.end()) cache.erase(it);
   }
   lru.push_front(key);
   p->lru_pos = lru.begin();
@@ -152,7 +148,7 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   const bool fits32 = d.M < (int64_t(1) << 31) && d.N < (int64_t(1) << 31) && d.K < (int64_t(1) << 31);
   const bool real_fp = true;
   if (precision != MUSCLE_HIP_PRECISION_NATIVE && d.has_b && is_double(d.dt[2]) && d.M >= 16 && d.N >= 16 &&
-      d.K >= 16 && fits32) {
+      d.K >= 16 && fits32 && d.B <= 65535) {
     p->kind = Plan::SPLIT;
   } else if (d.has_b && real_fp && fits32 && d.M >= 32 && d.N >= 32 && d.K >= 8) {
     p->kind = Plan::GEMM;
@@ -237,7 +233,8 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
   std::memset(info, 0, sizeof *info);
   const ContractionDesc& d = p.d;
   info->kernel = p.kind == Plan::GEMM ? 1 : p.kind == Plan::REDUCE ? 2 : p.kind == Plan::SPLIT ? 3 : 0;
-  info->precision = p.kind == Plan::SPLIT ? p.precision : MUSCLE_HIP_PRECISION_NATIVE;
+  info->precision = p.fp32_math ? MUSCLE_HIP_PRECISION_FP32
+                                 : (p.kind == Plan::SPLIT ? p.precision : MUSCLE_HIP_PRECISION_NATIVE);
   info->slices = p.slices;
   info->swapped = d.swapped;
   info->m = d.M;
@@ -312,7 +309,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
       g.bA = T + at[6] + b0;
       g.bB = T + at[7] + b0;
       g.bC = T + at[8] + b0;
-      launch_gemm_mfma(dbl, is_complex(dta), is_complex(dtb), g, h.stream);
+      launch_gemm_mfma(dbl, p.fp32_math, is_complex(dta), is_complex(dtb), g, h.stream);
       check_launch();
     }
     return;
@@ -359,6 +356,7 @@ Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_h
   std::unique_ptr<Plan> np = build_contraction_plan(d, precision);
   if (np->kind == Plan::SPLIT) prepare_split(*np, target);
   if (np->kind != Plan::EMPTY) upload_tables(*np, h.stream);
+  upload_split_tables(*np, h.stream);
   return h.insert_plan(key, std::move(np));
 }
 
@@ -468,8 +466,6 @@ int muscle_hip_plan_cache_clear(muscle_hip_handle_t handle) {
     if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
     std::lock_guard<std::mutex> lk(h->mu);
     MH_HIP_CHECK(hipDeviceSynchronize());
-    for (auto& kv : h->cache)
-      if (kv.second->dtab) (void)hipFree(kv.second->dtab);
     h->cache.clear();
     h->lru.clear();
   });
@@ -554,15 +550,16 @@ int muscle_hip_hadamard(muscle_hip_handle_t handle, const muscle_hip_tensor_t* c
 }
 
 int muscle_hip_describe(const muscle_hip_tensor_t* c, const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b,
-                        int precision, muscle_hip_info_t* info, int32_t* modes_out, int32_t* class_out,
-                        int32_t* nmodes_out) {
+                        int precision, double target_rel_error, muscle_hip_info_t* info, int32_t* modes_out,
+                        int32_t* class_out, int32_t* nmodes_out) {
   return guarded([&] {
     check_tensor(c, "C", false);
     check_tensor(a, "A", false);
     check_tensor(b, "B", false);
     ContractionDesc d = lower_binary(*c, *a, *b);
     std::unique_ptr<Plan> p = build_contraction_plan(d, precision);
-    if (p->kind == Plan::SPLIT) prepare_split(*p, 1e-12);
+    if (!(target_rel_error > 0)) target_rel_error = 1e-12;
+    if (p->kind == Plan::SPLIT) prepare_split(*p, target_rel_error);
     fill_info(info, *p, c, a, b);
     if (nmodes_out) *nmodes_out = (int32_t)d.modes.size();
     for (size_t i = 0; i < d.modes.size(); ++i) {

@@ -53,4 +53,20 @@ __device__ inline void store_elem(void* base, int64_t off, Cplx<T> v) {
   }
 }
 
+// Tile order of the GEMM kernels: the dispatcher deals workgroups
+// round-robin over the 8 XCDs, so workgroup L is renumbered to give each XCD
+// a contiguous range; the range is walked in groups of 8 tile rows, row
+// fastest, so workgroups running together share A and B panels in their L2.
+__device__ inline void gemm_tile_coords(int L, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
+  constexpr int GROUP_M = 8;
+  if ((nwg & 7) == 0) L = (L & 7) * (nwg >> 3) + (L >> 3);
+  const int per_group = GROUP_M * tiles_n;
+  const int g = L / per_group;
+  const int first_m = g * GROUP_M;
+  const int gsize = min(tiles_m - first_m, GROUP_M);
+  const int r = L - g * per_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
 }  // namespace mh

@@ -57,26 +57,12 @@ struct Mfma<float> {
 
 constexpr int BM = 64, BN = 64, NT = 256;
 constexpr int LDP = 16;  // LDS row padding (elements)
-constexpr int GROUP_M = 8;
 
-}  // namespace
 
-// Tile order: XCD-contiguous ranges, then groups of GROUP_M tile rows walked
-// row-fastest so concurrently running workgroups share A and B panels.
-__device__ void gemm_tile_coords(int L, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
-  if ((nwg & 7) == 0) L = (L & 7) * (nwg >> 3) + (L >> 3);
-  const int per_group = GROUP_M * tiles_n;
-  const int g = L / per_group;
-  const int first_m = g * GROUP_M;
-  const int gsize = min(tiles_m - first_m, GROUP_M);
-  const int r = L - g * per_group;
-  tm = first_m + r % gsize;
-  tn = r / gsize;
-}
-
-namespace {
-
-template <typename T, bool CA, bool CB, bool G3, int BK>
+// T: arithmetic type (MFMA precision); S: storage type of A, B and C.  S = T
+// is the plain path; S = double with T = float is the FP32 rung of the
+// precision ladder (operands rounded to FP32 as they are gathered).
+template <typename T, typename S, bool CA, bool CB, bool G3, int BK>
 __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   using M_ = Mfma<T>;
   using acc_t = typename M_::acc_t;
@@ -145,8 +131,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   auto gather = [&](int kt) {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
-      ra[i] = load_elem<T, CA>(p.A, rowA_r[i] + tA[i]);
-      rb[i] = load_elem<T, CB>(p.B, colB_r[i] + tB[i]);
+      const Cplx<S> xa = load_elem<S, CA>(p.A, rowA_r[i] + tA[i]);
+      const Cplx<S> xb = load_elem<S, CB>(p.B, colB_r[i] + tB[i]);
+      ra[i] = Cplx<T>{(T)xa.re, (T)xa.im};
+      rb[i] = Cplx<T>{(T)xb.re, (T)xb.im};
       const bool ka = kbeg + kt * BK + a_kk[i] < kend, kb = kbeg + kt * BK + b_kk[i] < kend;
       if (!(a_ok[i] && ka)) ra[i] = Cplx<T>{T(0), T(0)};
       if (!(b_ok[i] && kb)) rb[i] = Cplx<T>{T(0), T(0)};
@@ -252,13 +240,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
           im = acc1[i][j][r];
         }
         if (to_partial) store_elem<T, CC>(p.partial, cn + (int64_t)m * p.N, Cplx<T>{re, im});
-        else store_elem<T, CC>(p.C, cn + p.rowC[m], Cplx<T>{re, im});
+        else store_elem<S, CC>(p.C, cn + p.rowC[m], Cplx<S>{(S)re, (S)im});
       }
     }
 }
 
 // C[bz, m, n] = sum over slices s of partial[s][bz][m][n], in slice order.
-template <typename T, bool CC>
+template <typename T, typename S, bool CC>
 __global__ __launch_bounds__(NT) void splitk_finish_kernel(GemmArgs p) {
   const int64_t mn = (int64_t)p.M * p.N;
   const int64_t total = mn * p.batch;
@@ -273,23 +261,23 @@ __global__ __launch_bounds__(NT) void splitk_finish_kernel(GemmArgs p) {
   const int bz = (int)(o / mn);
   const int64_t r = o - bz * mn;
   const int m = (int)(r / p.N), n = (int)(r - (int64_t)m * p.N);
-  store_elem<T, CC>(p.C, p.bC[bz] + p.rowC[m] + p.colC[n], Cplx<T>{re, im});
+  store_elem<S, CC>(p.C, p.bC[bz] + p.rowC[m] + p.colC[n], Cplx<S>{(S)re, (S)im});
 }
 
-template <typename T, bool CA, bool CB, bool G3>
+template <typename T, typename S, bool CA, bool CB, bool G3>
 void launch_t(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.batch, (unsigned)g.ksplit);
-  hipLaunchKernelGGL((gemm_tile_kernel<T, CA, CB, G3, 16>), grid, dim3(NT), 0, s, g);
+  hipLaunchKernelGGL((gemm_tile_kernel<T, S, CA, CB, G3, 16>), grid, dim3(NT), 0, s, g);
   if (g.ksplit > 1) {
     const int64_t total = (int64_t)g.M * g.N * g.batch;
-    hipLaunchKernelGGL((splitk_finish_kernel<T, CA || CB>), dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0, s,
-                       g);
+    hipLaunchKernelGGL((splitk_finish_kernel<T, S, CA || CB>), dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0,
+                       s, g);
   }
 }
 
 }  // namespace
 
-void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g0, hipStream_t s) {
+void launch_gemm_mfma(bool dbl, bool fp32_math, bool ca, bool cb, const GemmArgs& g0, hipStream_t s) {
   GemmArgs g = g0;
   g.tiles_m = (g.M + BM - 1) / BM;
   g.tiles_n = (g.N + BN - 1) / BN;
@@ -299,17 +287,27 @@ void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g0, hipStream_
   }
   const bool g3 = ca && cb && g.complex3m;
   const int key = (dbl ? 8 : 0) | (ca ? 4 : 0) | (cb ? 2 : 0) | (g3 ? 1 : 0);
+  if (dbl && fp32_math) {
+    switch (key) {
+      case 8: launch_t<float, double, false, false, false>(g, s); break;
+      case 10: launch_t<float, double, false, true, false>(g, s); break;
+      case 12: launch_t<float, double, true, false, false>(g, s); break;
+      case 14: launch_t<float, double, true, true, false>(g, s); break;
+      case 15: launch_t<float, double, true, true, true>(g, s); break;
+    }
+    return;
+  }
   switch (key) {
-    case 0: launch_t<float, false, false, false>(g, s); break;
-    case 2: launch_t<float, false, true, false>(g, s); break;
-    case 4: launch_t<float, true, false, false>(g, s); break;
-    case 6: launch_t<float, true, true, false>(g, s); break;
-    case 7: launch_t<float, true, true, true>(g, s); break;
-    case 8: launch_t<double, false, false, false>(g, s); break;
-    case 10: launch_t<double, false, true, false>(g, s); break;
-    case 12: launch_t<double, true, false, false>(g, s); break;
-    case 14: launch_t<double, true, true, false>(g, s); break;
-    case 15: launch_t<double, true, true, true>(g, s); break;
+    case 0: launch_t<float, float, false, false, false>(g, s); break;
+    case 2: launch_t<float, float, false, true, false>(g, s); break;
+    case 4: launch_t<float, float, true, false, false>(g, s); break;
+    case 6: launch_t<float, float, true, true, false>(g, s); break;
+    case 7: launch_t<float, float, true, true, true>(g, s); break;
+    case 8: launch_t<double, double, false, false, false>(g, s); break;
+    case 10: launch_t<double, double, false, true, false>(g, s); break;
+    case 12: launch_t<double, double, true, false, false>(g, s); break;
+    case 14: launch_t<double, double, true, true, false>(g, s); break;
+    case 15: launch_t<double, double, true, true, true>(g, s); break;
   }
 }
 

@@ -25,6 +25,7 @@ struct Plan {
   std::vector<size_t> tab_at;
   int64_t* dtab = nullptr;
   bool a_kmajor = false, b_kmajor = false;
+  bool fp32_math = false;  // FP64 operands multiplied on FP32 MFMA (precision FP32)
   // tile GEMM split-K
   int gemm_ksplit = 1;
   int gemm_kchunk = 0;
@@ -32,11 +33,28 @@ struct Plan {
   int gshift = 0;
   int nsplit = 1;
   int64_t kchunk = 1;
-  // split-precision configuration
+  // split-precision configuration (sliced modes)
   int slices = 0;
-  int64_t kblock = 0;  // K-chunk whose FP32 partial sums are exact / combined in FP64
+  int split_mode = -1;  // 0 int8 base-255 digits (Ozaki), 1 bf16 x3, 2 fp16 x2
+  int planes_a = 0, planes_b = 0;
+  int64_t Mp = 0, Np = 0, Kp = 0;  // padded extents of the packed operands
+  int esize = 1;                   // bytes per packed element
+  int ngroups = 0;
+  std::vector<int32_t> split_segs;    // SliceSeg records, flattened
+  std::vector<int32_t> split_groups;  // SliceGroup records, flattened
+  std::vector<double> split_coef;     // [2][ngroups]: real / imaginary combine weights
+  void* split_dev = nullptr;          // device copy: segs | groups | coef
+  size_t dev_groups_off = 0, dev_coef_off = 0;
   double err_bound = 0.0;
   std::list<std::string>::iterator lru_pos;
+
+  Plan() = default;
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+  ~Plan() {
+    if (dtab) (void)hipFree(dtab);
+    if (split_dev) (void)hipFree(split_dev);
+  }
 };
 
 struct Handle {

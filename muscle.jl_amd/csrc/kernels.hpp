@@ -55,8 +55,9 @@ struct GemmArgs {
   int32_t ksplit, kchunk;
   void* partial;
 };
-// dbl: FP64 (else FP32) arithmetic; ca/cb: operand is complex.
-void launch_gemm_mfma(bool dbl, bool ca, bool cb, const GemmArgs& g, hipStream_t s);
+// dbl: FP64 storage (else FP32); fp32_math: FP64 storage with FP32 MFMA
+// arithmetic (the FP32 rung); ca/cb: operand is complex.
+void launch_gemm_mfma(bool dbl, bool fp32_math, bool ca, bool cb, const GemmArgs& g, hipStream_t s);
 
 // ---- reduction kernel (skinny contractions, unary sums, traces) -------------
 struct ReduceArgs {

@@ -84,7 +84,7 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_binary_einsum": ([H, T, T, T, ctypes.c_int, ctypes.c_double, I], ctypes.c_int),
         "muscle_hip_hadamard": ([H, T, T, T, I], ctypes.c_int),
         "muscle_hip_unary_einsum": ([H, T, T, I], ctypes.c_int),
-        "muscle_hip_describe": ([T, T, T, ctypes.c_int, I, ctypes.POINTER(ctypes.c_int32),
+        "muscle_hip_describe": ([T, T, T, ctypes.c_int, ctypes.c_double, I, ctypes.POINTER(ctypes.c_int32),
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

@@ -9,7 +9,7 @@ CLASS_NAMES = {0: "batch", 1: "left", 2: "right", 3: "contract", 4: "sum_a", 5: 
 
 
 def describe(inds_c, shape_c, inds_a, shape_a, inds_b, shape_b, dtypes=("c64", "c64", "c64"),
-             strides=(None, None, None), precision: str = "native") -> dict:
+             strides=(None, None, None), precision: str = "native", target_rel_error: float = 1e-12) -> dict:
     """Classify every index and report the fused GEMM shape and kernel choice.
 
     Index labels may be any hashables; shapes are per-index extents; strides
@@ -33,7 +33,8 @@ def describe(inds_c, shape_c, inds_a, shape_a, inds_b, shape_b, dtypes=("c64", "
     classes = (ctypes.c_int32 * nmax)()
     nm = ctypes.c_int32()
     _lib.check(_lib.lib.muscle_hip_describe(descs[0].ref, descs[1].ref, descs[2].ref,
-                                            _lib.PRECISION[precision], ctypes.byref(info), modes, classes,
+                                            _lib.PRECISION[precision], float(target_rel_error),
+                                            ctypes.byref(info), modes, classes,
                                             ctypes.byref(nm)))
     inv = {v: k for k, v in labels.items()}
     out = info.as_dict()

@@ -1,0 +1,90 @@
+"""The precision ladder of binary_einsum for Float64 / ComplexF64 operands,
+against the FP64 CPU oracle (BackendBase restatement).
+
+Tolerances (norm-wise relative, ||C - C_oracle|| / ||C_oracle||):
+  fp32        2e-5   (FP32 operands and accumulation; K up to 1536)
+  bf16x3      2e-5   (three bf16 terms, FP32 accumulation per 1024-long K chunk)
+  fp16x2      2e-4   (two fp16 terms of row/column-scaled values)
+  ozaki_int8  the requested target: 1e-10 (5 base-255 digits) and 1e-12 (6)
+"""
+import numpy as np
+import pytest
+
+import muscle
+from muscle.workloads import ladder_inputs
+from oracle import backend_base as bb
+from tests.gpu_util import dev_tensor, host, randn, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 2e-5, "bf16x3": 2e-5, "fp16x2": 2e-4}
+
+SHAPES = [
+    ("config1", ["i", "j", "k"], (64, 32, 48), ["k", "l", "j"], (48, 96, 32), ["i", "l"]),
+    ("ragged", ["i", "j"], (70, 45), ["j", "k"], (45, 93), ["k", "i"]),
+    ("hyper", ["i", "j", "h"], (40, 36, 3), ["j", "k", "h"], (36, 50, 3), ["h", "k", "i"]),
+    ("interleaved d=8", ["a1", "c1", "a3", "c2", "a5", "c3"], (8,) * 6, ["c1", "b1", "c2", "b2", "c3", "b3"], (8,) * 6,
+     ["a1", "b1", "a3", "b2", "a5", "b3"]),
+]
+
+
+def run(A, ia, B, ib, ic, **backend):
+    with muscle.with_backend(muscle.BackendHIP(**backend)):
+        c = muscle.binary_einsum(dev_tensor(A, ia), dev_tensor(B, ib), out=ic)
+    return host(c), muscle.last_info()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("T", [np.complex128, np.float64])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: s[0])
+def test_ladder_rungs(shape, T, mode):
+    name, ia, sa, ib, sb, ic = shape
+    rng = np.random.default_rng(5)
+    A, B = randn(rng, sa, T), randn(rng, sb, T)
+    ref = bb.binary_einsum_general(ic, A, ia, B, ib)
+    C, info = run(A, ia, B, ib, ic, precision=mode)
+    assert C.dtype == T
+    assert info["precision"] == muscle._lib.PRECISION[mode]
+    assert rel_err(C, ref) <= TOL[mode], (mode, rel_err(C, ref))
+
+
+@pytest.mark.parametrize("target, S", [(1e-10, 5), (1e-12, 6)])
+@pytest.mark.parametrize("T", [np.complex128, np.float64])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: s[0])
+def test_ozaki_int8(shape, T, target, S):
+    name, ia, sa, ib, sb, ic = shape
+    rng = np.random.default_rng(6)
+    A, B = randn(rng, sa, T), randn(rng, sb, T)
+    ref = bb.binary_einsum_general(ic, A, ia, B, ib)
+    C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=target)
+    assert info["kernel"] == 3 and info["slices"] == S
+    assert rel_err(C, ref) <= target, rel_err(C, ref)
+
+
+@pytest.mark.parametrize("ta, tb", [(np.float64, np.complex128), (np.complex128, np.float64)])
+def test_ozaki_mixed_real_complex(ta, tb):
+    rng = np.random.default_rng(7)
+    A, B = randn(rng, (70, 90), ta), randn(rng, (90, 60), tb)
+    ref = A @ B
+    C, info = run(A, ["i", "j"], B, ["j", "k"], ["i", "k"], precision="ozaki_int8", target_rel_error=1e-10)
+    assert info["kernel"] == 3
+    assert rel_err(C, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("dist", ["normal", "loguniform", "unitrows"])
+def test_ladder_distributions_config1(dist):
+    """config-1 shapes under the three value distributions of the precision ladder."""
+    A, ia, B, ib, dims, ic = ladder_inputs(dist, 0)
+    ref = bb.binary_einsum_base(ic, A, ia, B, ib)
+    C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=1e-10)
+    assert rel_err(C, ref) <= 1e-10, rel_err(C, ref)
+    C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=1e-12)
+    assert rel_err(C, ref) <= 1e-12, rel_err(C, ref)
+
+
+def test_float32_operands_ignore_ladder():
+    rng = np.random.default_rng(8)
+    A, B = randn(rng, (64, 64), np.complex64), randn(rng, (64, 64), np.complex64)
+    C, info = run(A, ["i", "j"], B, ["j", "k"], ["i", "k"], precision="ozaki_int8")
+    assert info["precision"] == 0  # FP32 data: native arithmetic
+    assert rel_err(C, A.astype(np.complex128) @ B.astype(np.complex128)) <= 2e-5

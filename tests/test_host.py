@@ -103,6 +103,22 @@ def test_promotion_rule_enforced():
     assert d["flops"] == 4 * 2 * 4 * 3
 
 
+@pytest.mark.parametrize("target, S", [(1e-5, 3), (1e-7, 4), (1e-10, 5), (1e-12, 6), (1e-14, 7)])
+def test_ozaki_slice_count_from_target(target, S):
+    """Smallest S with 0.4 * 255^-(S-1) <= target (the calibrated base-255 error model)."""
+    shapes = (["i", "k"], [64, 64], ["i", "j"], [64, 128], ["j", "k"], [128, 64])
+    d = describe(*shapes, precision="ozaki_int8", target_rel_error=target)
+    assert d["kernel_name"] == "split" and d["slices"] == S
+    assert d["err_bound"] <= target
+
+
+def test_ladder_modes_lowering():
+    for mode in ("fp32", "bf16x3", "fp16x2"):
+        d = describe(["i", "k"], [64, 64], ["i", "j"], [64, 128], ["j", "k"], [128, 64], precision=mode)
+        assert d["kernel_name"] == ("gemm_mfma" if mode == "fp32" else "split")
+        assert d["precision"] == muscle._lib.PRECISION[mode] or mode == "fp32"
+
+
 def test_strided_view_lowering():
     # A is a view with non-unit strides: (i stride 2, j stride 20)
     d = describe(["i", "k"], [5, 6], ["i", "j"], [5, 7], ["j", "k"], [7, 6],
