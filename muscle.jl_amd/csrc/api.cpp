@@ -124,17 +124,24 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
     p->kind = Plan::EMPTY;
     return p;
   }
-  std::vector<size_t> at;
-  append_table(p->htab, at, offset_table(d.m, 0));      // 0 rowA
-  append_table(p->htab, at, offset_table(d.k, 0));      // 1 kA
-  append_table(p->htab, at, offset_table(d.k, 1));      // 2 kB
-  append_table(p->htab, at, offset_table(d.n, 1));      // 3 colB
-  append_table(p->htab, at, offset_table(d.m, 2));      // 4 rowC
-  append_table(p->htab, at, offset_table(d.n, 2));      // 5 colC
-  append_table(p->htab, at, offset_table(d.batch, 0));  // 6 bA
-  append_table(p->htab, at, offset_table(d.batch, 1));  // 7 bB
-  append_table(p->htab, at, offset_table(d.batch, 2));  // 8 bC
-  p->tab_at = at;
+  const int64_t lim32 = int64_t(1) << 32;
+  if (d.M >= lim32 || d.N >= lim32 || d.K >= lim32 || d.B >= lim32)
+    fail(MUSCLE_HIP_UNSUPPORTED, "an index class with 2^32 or more elements");
+  // two-level tables: (class dims, operand slot) per TabId
+  const std::vector<Dim>* cls[TAB_COUNT] = {&d.m, &d.k, &d.k, &d.n, &d.m, &d.n, &d.batch, &d.batch, &d.batch};
+  const int slot[TAB_COUNT] = {0, 0, 1, 1, 2, 2, 0, 1, 2};
+  for (int t = 0; t < TAB_COUNT; ++t) {
+    std::vector<Dim> lo, hi;
+    int64_t P;
+    split_dims(*cls[t], lo, hi, P);
+    std::vector<size_t> at;
+    append_table(p->htab, at, offset_table(lo, slot[t]));
+    append_table(p->htab, at, offset_table(hi, slot[t]));
+    p->tab_lo[t] = at[0];
+    p->tab_hi[t] = at[1];
+    p->tab_P[t] = (uint32_t)P;
+    p->tab_single[t] = hi.empty() ? 1u : 0u;
+  }
 
   auto min_stride = [](const std::vector<Dim>& v, int w) {
     int64_t b = INT64_MAX;
@@ -275,8 +282,6 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   int dta, dtb = d.dt[1];
   const void* A = promote_operand(h, *ka, dbl, cursor, dta);
   const void* B = kb ? promote_operand(h, *kb, dbl, cursor, dtb) : nullptr;
-  const int64_t* T = p.dtab;
-  const auto& at = p.tab_at;
   if (p.kind == Plan::SPLIT) {
     run_split(h, p, c.data, A, B, is_complex(dta), is_complex(dtb), (char*)h.ws + split_off);
     check_launch();
@@ -288,12 +293,12 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     g.A = A;
     g.B = B;
     g.C = c.data;
-    g.rowA = T + at[0];
-    g.kA = T + at[1];
-    g.kB = T + at[2];
-    g.colB = T + at[3];
-    g.rowC = T + at[4];
-    g.colC = T + at[5];
+    g.rowA = p.tab(TAB_ROWA);
+    g.kA = p.tab(TAB_KA);
+    g.kB = p.tab(TAB_KB);
+    g.colB = p.tab(TAB_COLB);
+    g.rowC = p.tab(TAB_ROWC);
+    g.colC = p.tab(TAB_COLC);
     g.M = (int32_t)d.M;
     g.N = (int32_t)d.N;
     g.K = (int32_t)d.K;
@@ -306,9 +311,9 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     // grid.y carries the batch; split launches beyond the 65535 limit
     for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
       g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);
-      g.bA = T + at[6] + b0;
-      g.bB = T + at[7] + b0;
-      g.bC = T + at[8] + b0;
+      g.bA = p.tab(TAB_BA, (uint32_t)b0);
+      g.bB = p.tab(TAB_BB, (uint32_t)b0);
+      g.bC = p.tab(TAB_BC, (uint32_t)b0);
       launch_gemm_mfma(dbl, p.fp32_math, is_complex(dta), is_complex(dtb), g, h.stream);
       check_launch();
     }
@@ -320,15 +325,15 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   r.B = B;
   r.C = c.data;
   r.partial = (char*)h.ws + partial_off;
-  r.rowA = T + at[0];
-  r.kA = T + at[1];
-  r.kB = T + at[2];
-  r.colB = T + at[3];
-  r.rowC = T + at[4];
-  r.colC = T + at[5];
-  r.bA = T + at[6];
-  r.bB = T + at[7];
-  r.bC = T + at[8];
+  r.rowA = p.tab(TAB_ROWA);
+  r.kA = p.tab(TAB_KA);
+  r.kB = p.tab(TAB_KB);
+  r.colB = p.tab(TAB_COLB);
+  r.rowC = p.tab(TAB_ROWC);
+  r.colC = p.tab(TAB_COLC);
+  r.bA = p.tab(TAB_BA);
+  r.bB = p.tab(TAB_BB);
+  r.bC = p.tab(TAB_BC);
   r.M = d.M;
   r.N = d.N;
   r.K = d.K;

@@ -10,9 +10,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "kernels.hpp"
 #include "lowering.hpp"
 
 namespace mh {
+
+// Offset tables of a plan, in this order; each has a lo and a hi part.
+enum TabId { TAB_ROWA = 0, TAB_KA, TAB_KB, TAB_COLB, TAB_ROWC, TAB_COLC, TAB_BA, TAB_BB, TAB_BC, TAB_COUNT };
 
 // A lowered contraction with its offset tables resident on the device.
 struct Plan {
@@ -20,10 +24,22 @@ struct Plan {
   int kind = EMPTY;
   int precision = MUSCLE_HIP_PRECISION_NATIVE;
   ContractionDesc d;
-  // offset tables: rowA kA kB colB rowC colC bA bB bC, concatenated
+  // two-level offset tables (lo, hi for each TabId), concatenated; P of each
   std::vector<int64_t> htab;
-  std::vector<size_t> tab_at;
+  size_t tab_lo[TAB_COUNT] = {}, tab_hi[TAB_COUNT] = {};
+  uint32_t tab_P[TAB_COUNT] = {};
+  uint32_t tab_single[TAB_COUNT] = {};
   int64_t* dtab = nullptr;
+  // device view of one table (valid once uploaded)
+  OffTab tab(int id, uint32_t base = 0) const {
+    OffTab t;
+    t.lo = dtab + tab_lo[id];
+    t.hi = dtab + tab_hi[id];
+    t.div = FastDiv(tab_P[id]);
+    t.base = base;
+    t.single = tab_single[id];
+    return t;
+  }
   bool a_kmajor = false, b_kmajor = false;
   bool fp32_math = false;  // FP64 operands multiplied on FP32 MFMA (precision FP32)
   // tile GEMM split-K

@@ -30,20 +30,33 @@ struct FastDiv {
   }
 };
 
+// Two-level offset table of one index class for one operand:
+//   off(i) = lo[i mod P] + hi[i div P]
+// where the class's (fused) modes are split into a fast part of product P,
+// about the square root of the class size, and a slow part.  Tables stay
+// O(sqrt(n)) in size: a 2^28-element output mode class needs 2 x 16384 entries.
+// Classes of up to 65536 entries keep a single level (hi = {0}, single = 1),
+// read with one load.
+struct OffTab {
+  const int64_t* lo;
+  const int64_t* hi;
+  FastDiv div;      // divisor P
+  uint32_t base;    // added to every index (batch chunks)
+  uint32_t single;  // one-level table: off(i) = lo[i]
+  __device__ inline int64_t operator[](uint32_t i) const {
+    i += base;
+    if (single) return lo[i];
+    const uint32_t q = div.div(i);
+    return lo[i - q * div.d] + hi[q];
+  }
+};
+
 // ---- tile GEMM on MFMA ----------------------------------------------------
 struct GemmArgs {
   const void* A;
   const void* B;
   void* C;
-  const int64_t* rowA;  // [M]
-  const int64_t* kA;    // [K]
-  const int64_t* kB;    // [K]
-  const int64_t* colB;  // [N]
-  const int64_t* rowC;  // [M]
-  const int64_t* colC;  // [N]
-  const int64_t* bA;    // [batch]
-  const int64_t* bB;
-  const int64_t* bC;
+  OffTab rowA, kA, kB, colB, rowC, colC, bA, bB, bC;
   int32_t M, N, K, batch;
   int32_t a_kmajor;  // A's contiguous direction is K (else M)
   int32_t b_kmajor;  // B's contiguous direction is K (else N)
@@ -65,15 +78,7 @@ struct ReduceArgs {
   const void* B;  // null when has_b == false
   void* C;
   void* partial;  // [nsplit][outputs] when nsplit > 1
-  const int64_t* rowA;
-  const int64_t* kA;
-  const int64_t* kB;
-  const int64_t* colB;
-  const int64_t* rowC;
-  const int64_t* colC;
-  const int64_t* bA;
-  const int64_t* bB;
-  const int64_t* bC;
+  OffTab rowA, kA, kB, colB, rowC, colC, bA, bB, bC;
   int64_t M, N, K, batch, outputs;
   int64_t kchunk;
   int32_t gshift;  // lanes per output = 1 << gshift (<= 64)

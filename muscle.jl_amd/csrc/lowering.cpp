@@ -13,6 +13,7 @@
 #include "lowering.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 
@@ -115,6 +116,52 @@ int64_t product(const std::vector<Dim>& d) {
 }
 
 }  // namespace
+
+void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim>& hi, int64_t& P) {
+  lo.clear();
+  hi.clear();
+  int64_t total = product(d);
+  P = 1;
+  if (total <= 1) {
+    hi = d;
+    return;
+  }
+  if (total <= kSingleLevelMax) {  // small class: one level, P = total
+    lo = d;
+    P = total;
+    return;
+  }
+  const double target = std::sqrt((double)total);
+  size_t i = 0;
+  for (; i < d.size(); ++i) {
+    const Dim& x = d[i];
+    if ((double)P * (double)x.extent <= target) {
+      lo.push_back(x);
+      P *= x.extent;
+      continue;
+    }
+    // split x = (f, s) x (e / f, s * f) with P * f as close to target as possible
+    int64_t best = 1;
+    const int64_t lim = (int64_t)(target / (double)P);
+    for (int64_t f = std::min<int64_t>(lim, x.extent - 1); f >= 2; --f)
+      if (x.extent % f == 0) {
+        best = f;
+        break;
+      }
+    if (best > 1) {
+      Dim a = x, b = x;
+      a.extent = best;
+      b.extent = x.extent / best;
+      for (int w = 0; w < 3; ++w) b.s[w] = x.s[w] * best;
+      lo.push_back(a);
+      hi.push_back(b);
+      P *= best;
+      ++i;
+    }
+    break;
+  }
+  for (; i < d.size(); ++i) hi.push_back(d[i]);
+}
 
 std::vector<int64_t> offset_table(const std::vector<Dim>& dims, int which) {
   int64_t n = product(dims);

@@ -53,6 +53,13 @@ ContractionDesc lower_unary(const muscle_hip_tensor_t& y, const muscle_hip_tenso
 ElementwiseDesc lower_hadamard(const muscle_hip_tensor_t& c, const muscle_hip_tensor_t& a,
                                const muscle_hip_tensor_t& b);
 
+// Splits a fastest-first dim list into a fast part (lo) of product P close to
+// sqrt(product) and a slow part (hi), splitting one dim at a factor boundary
+// when that gets closer; offsets then factor as lo-offset + hi-offset.
+// Classes of at most kSingleLevelMax points are not split (lo = d, P = total).
+constexpr int64_t kSingleLevelMax = 65536;
+void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim>& hi, int64_t& P);
+
 // Offsets of every point of a mixed-radix loop nest (fastest-first dims),
 // using stride slot `which` (0 A, 1 B, 2 C).  Length = product of extents
 // (1 for an empty list).

@@ -43,9 +43,7 @@ enum { MODE_I8 = 0, MODE_BF16 = 1, MODE_F16 = 2 };
 // ------------------------------------------------------------------ scale
 struct ScaleArgs {
   const void* X;
-  const int64_t* rowX;
-  const int64_t* kX;
-  const int64_t* bX;
+  OffTab rowX, kX, bX;
   int32_t* exps;  // [B][R]
   int32_t R, K, B, cplx, bias;
 };
@@ -70,9 +68,7 @@ __global__ __launch_bounds__(NT) void scale_kernel(ScaleArgs p) {
 // ------------------------------------------------------------------ pack
 struct PackArgs {
   const void* X;
-  const int64_t* rowX;
-  const int64_t* kX;
-  const int64_t* bX;
+  OffTab rowX, kX, bX;
   const int32_t* exps;  // [B][R] or null (no scaling)
   void* Q;              // [planes*S][B][Rp][Kp] elements
   int32_t R, K, B, Rp, Kp;
@@ -324,9 +320,7 @@ struct CombineArgs {
   const int32_t* expA;  // [B][M] or null
   const int32_t* expB;  // [B][N] or null
   void* C;
-  const int64_t* rowC;
-  const int64_t* colC;
-  const int64_t* bC;
+  OffTab rowC, colC, bC;
   int32_t G, B, Mp, Np, M, N, is_int, c_complex;
 };
 
@@ -519,24 +513,24 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
   const ContractionDesc& d = p.d;
   const WsLayout w = ws_layout(p);
   char* base = (char*)ws;
-  const int64_t* T = p.dtab;
-  const auto& at = p.tab_at;
   int32_t* expA = (int32_t*)(base + w.expA);
   int32_t* expB = (int32_t*)(base + w.expB);
   const bool scaled = p.split_mode != MODE_BF16;
   const int bias = p.split_mode == MODE_I8 ? 2 : 1;
   hipStream_t s = h.stream;
   if (scaled) {
-    ScaleArgs sa{A, T + at[0], T + at[1], T + at[6], expA, (int32_t)d.M, (int32_t)d.K, (int32_t)d.B, ca, bias};
+    ScaleArgs sa{A, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), expA, (int32_t)d.M, (int32_t)d.K, (int32_t)d.B, ca,
+                 bias};
     const int64_t rows = d.M * d.B;
     hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(NT), 0, s, sa);
-    ScaleArgs sb{B, T + at[3], T + at[2], T + at[7], expB, (int32_t)d.N, (int32_t)d.K, (int32_t)d.B, cb, bias};
+    ScaleArgs sb{B, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), expB, (int32_t)d.N, (int32_t)d.K, (int32_t)d.B, cb,
+                 bias};
     const int64_t cols = d.N * d.B;
     hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((cols + 3) / 4)), dim3(NT), 0, s, sb);
   }
   auto launch_pack = [&](const void* X, int rowt, int kt, int bt, int32_t* exps, void* Q, int64_t R, int64_t Rp,
                          bool cplx, bool r_major, int planes) {
-    PackArgs pa{X, T + at[rowt], T + at[kt], T + at[bt], scaled ? exps : nullptr, Q, (int32_t)R, (int32_t)d.K,
+    PackArgs pa{X, p.tab(rowt), p.tab(kt), p.tab(bt), scaled ? exps : nullptr, Q, (int32_t)R, (int32_t)d.K,
                 (int32_t)d.B, (int32_t)Rp, (int32_t)p.Kp, cplx, r_major, planes, p.slices, p.split_mode};
     dim3 grid((unsigned)(p.Kp / PK), (unsigned)(Rp / PR), (unsigned)d.B);
     switch (p.split_mode) {
@@ -546,8 +540,8 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
     }
   };
   // A rows are M, its "row-major" gather direction is M when A is not K-major
-  launch_pack(A, 0, 1, 6, expA, base + w.QA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a);
-  launch_pack(B, 3, 2, 7, expB, base + w.QB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b);
+  launch_pack(A, TAB_ROWA, TAB_KA, TAB_BA, expA, base + w.QA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a);
+  launch_pack(B, TAB_COLB, TAB_KB, TAB_BB, expB, base + w.QB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b);
 
   const int64_t rowbytes = p.Kp * p.esize;
   SlicedArgs ga;
@@ -579,9 +573,9 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
                  scaled ? expA : nullptr,
                  scaled ? expB : nullptr,
                  C,
-                 T + at[4],
-                 T + at[5],
-                 T + at[8],
+                 p.tab(TAB_ROWC),
+                 p.tab(TAB_COLC),
+                 p.tab(TAB_BC),
                  p.ngroups,
                  (int32_t)d.B,
                  (int32_t)p.Mp,
