@@ -182,6 +182,66 @@ MUSCLE_HIP_API int muscle_hip_unary_einsum(muscle_hip_handle_t handle,
                                            const muscle_hip_tensor_t* x,
                                            muscle_hip_info_t* info);
 
+/* --- batched contractions (one persistent launch) ---------------------- */
+
+typedef struct muscle_hip_batch_s* muscle_hip_batch_t;
+
+/* Tile shapes of the batched kernel (rows of C x columns of C per workgroup);
+ * muscle_hip_batch_autotune picks one per size class. */
+typedef enum {
+  MUSCLE_HIP_TILE_AUTO = -1,
+  MUSCLE_HIP_TILE_64x64 = 0,
+  MUSCLE_HIP_TILE_32x128 = 1,
+  MUSCLE_HIP_TILE_128x32 = 2,
+  MUSCLE_HIP_TILE_COUNT = 3
+} muscle_hip_tile_t;
+
+typedef struct {
+  int64_t pairs;      /* contractions in the batch */
+  int64_t items;      /* output tiles the persistent launch works through */
+  int32_t workgroups; /* persistent grid size */
+  int32_t graph;      /* 1 once the launch has been captured in a hipGraph */
+  double flops;       /* sum of algorithmic flops */
+  double bytes;       /* sum of algorithmic bytes (every operand element once) */
+  int64_t table_bytes;/* device bytes of offset tables and pair descriptors */
+} muscle_hip_batch_info_t;
+
+/* Plans `n` independent binary contractions C_i = A_i * B_i, each with the
+ * semantics of muscle_hip_binary_einsum (batch, contracted, open, summed and
+ * diagonal modes), to run as ONE persistent kernel launch that works through
+ * the output tiles of all of them (largest first).  This replaces a loop of
+ * binary_einsum calls over the pairwise contractions of a contraction tree
+ * (the callers of src/Operations/binary_einsum.jl:33-51); the reference has
+ * no batched form.  All operands share one element type; NATIVE arithmetic.
+ * `tiles` (may be NULL) gives a muscle_hip_tile_t per pair; AUTO uses the
+ * handle's autotuned choice for the pair's size class, else a shape heuristic.
+ * The buffers must stay valid while the batch exists. */
+MUSCLE_HIP_API int muscle_hip_batch_create(muscle_hip_handle_t handle, int64_t n,
+                                           const muscle_hip_tensor_t* cs,
+                                           const muscle_hip_tensor_t* as,
+                                           const muscle_hip_tensor_t* bs,
+                                           const int32_t* tiles,
+                                           muscle_hip_batch_t* batch);
+/* mode 0: one persistent launch, captured in a hipGraph on the first run and
+ *         replayed afterwards;
+ * mode 1: one plain muscle_hip_binary_einsum launch per pair (the comparison
+ *         loop, no interpreter in between);
+ * mode 2: one persistent launch without graph capture. */
+MUSCLE_HIP_API int muscle_hip_batch_run(muscle_hip_handle_t handle, muscle_hip_batch_t batch, int mode);
+MUSCLE_HIP_API int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t* info);
+MUSCLE_HIP_API int muscle_hip_batch_destroy(muscle_hip_batch_t batch);
+
+/* Autotuner: for every size class present in the batch, times each tile shape
+ * on up to `samples` of its pairs (persistent launches, `reps` timed runs
+ * after a warm-up) and records the fastest in the handle's tuning table,
+ * which AUTO then uses.  The table can be saved/loaded as text. */
+MUSCLE_HIP_API int muscle_hip_batch_autotune(muscle_hip_handle_t handle, muscle_hip_batch_t batch, int samples,
+                                             int reps);
+MUSCLE_HIP_API int muscle_hip_tuning_save(muscle_hip_handle_t handle, const char* path);
+MUSCLE_HIP_API int muscle_hip_tuning_load(muscle_hip_handle_t handle, const char* path);
+/* Size class of a contraction of fused extents m x n (the key of the table). */
+MUSCLE_HIP_API int muscle_hip_tuning_class(int64_t m, int64_t n);
+
 /* Host-only: lower a binary contraction and report the fused GEMM shape and
  * the kernel the library would pick, without touching the GPU.  Replaces no
  * reference function; it exposes the index classification of

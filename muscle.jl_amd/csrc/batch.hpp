@@ -1,0 +1,77 @@
+// Batched contractions: many independent pairs in one persistent launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "handle.hpp"
+#include "kernels.hpp"
+
+namespace mh {
+
+// One contraction of the batch, as the persistent kernel reads it.
+struct BatchPair {
+  const void* A;
+  const void* B;
+  void* C;
+  int32_t M, N, K, NB;  // fused extents and batch (hyper) count
+  int32_t tiles_m, tiles_n, tile, a_kmajor, b_kmajor, pad;
+  OffTab rowA, kA, kB, colB, rowC, colC, bA, bB, bC;
+};
+
+// Output tile shapes: rows (M) x columns (N) per workgroup, and the 4 waves'
+// arrangement wm x wn; every wave holds 4 MFMA 16x16 tiles.
+struct TileShape {
+  int bm, bn, wm, wn;
+};
+constexpr TileShape kTileShapes[MUSCLE_HIP_TILE_COUNT] = {{64, 64, 2, 2}, {32, 128, 1, 4}, {128, 32, 4, 1}};
+
+struct BatchArgs {
+  const BatchPair* pairs;
+  const int2* items;  // (pair, tile within pair), largest pairs first
+  int32_t* counter;   // work queue head
+  int64_t nitems;
+  int32_t complex3m;
+};
+
+// Launches the persistent kernel (grid = workgroups) preceded by a one-thread
+// kernel that resets the work counter (no memset nodes: the launch is graph
+// captured).  lds_bytes: dynamic LDS for the largest tile shape present.
+void launch_batch(bool dbl, bool cplx, const BatchArgs& a, int workgroups, size_t lds_bytes, hipStream_t s);
+// Dynamic LDS one workgroup needs for a tile shape.
+size_t batch_lds_bytes(bool dbl, bool cplx, int tile);
+// Persistent workgroups per CU the kernel is built for.
+constexpr int kBatchWgPerCu = 2;
+
+struct Batch {
+  Handle* h = nullptr;
+  int dtype = 0;
+  std::vector<muscle_hip_tensor_t> cs, as, bs;  // for the per-pair plain loop
+  std::vector<std::vector<int32_t>> modes;      // storage behind cs/as/bs
+  std::vector<std::vector<int64_t>> ext, str;
+  std::vector<BatchPair> hpairs;
+  std::vector<int2> hitems;
+  std::vector<int64_t> htab;
+  std::vector<int> pair_class;
+  void* dmem = nullptr;  // pairs | items | counter | tables
+  BatchPair* dpairs = nullptr;
+  int2* ditems = nullptr;
+  int32_t* dcounter = nullptr;
+  int workgroups = 0;
+  size_t lds = 0;
+  double flops = 0, bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+  ~Batch();
+};
+
+std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
+                                   const muscle_hip_tensor_t* bs, const int32_t* tiles);
+void batch_run(Batch& b, int mode);
+void batch_autotune(Batch& b, int samples, int reps);
+int batch_tuning_class(int64_t m, int64_t n);
+
+}  // namespace mh

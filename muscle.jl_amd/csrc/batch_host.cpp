@@ -1,0 +1,408 @@
+// Host side of the batched contraction path: planning (lowering + compact
+// offset tables + work list), graph-captured persistent launches, the plain
+// per-pair loop it is compared with, and the tile-shape autotuner.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <numeric>
+
+#include "lowering.hpp"
+
+namespace mh {
+
+Batch::~Batch() {
+  if (h) (void)hipStreamSynchronize(h->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  if (dmem) (void)hipFree(dmem);
+}
+
+namespace {
+
+int bucket(int64_t x) { return x < 24 ? 0 : x < 48 ? 1 : x < 96 ? 2 : x < 192 ? 3 : 4; }
+
+int tuning_class(int64_t m, int64_t n) { return bucket(m) * 5 + bucket(n); }
+
+// Shape heuristic when nothing is tuned: keep most of the tile inside C.
+int default_tile(int64_t m, int64_t n) {
+  if (m < 48 && n >= 96) return MUSCLE_HIP_TILE_32x128;
+  if (n < 48 && m >= 96) return MUSCLE_HIP_TILE_128x32;
+  return MUSCLE_HIP_TILE_64x64;
+}
+
+size_t al(size_t v) { return (v + 255) & ~size_t(255); }
+
+struct TabRef {
+  size_t lo, hi;
+  uint32_t P, single;
+};
+
+TabRef add_table(std::vector<int64_t>& h, const std::vector<Dim>& dims, int slot) {
+  std::vector<Dim> lo, hi;
+  int64_t P;
+  split_dims(dims, lo, hi, P);
+  TabRef r;
+  const std::vector<int64_t> tl = offset_table(lo, slot), th = offset_table(hi, slot);
+  r.lo = h.size();
+  if (tl.empty()) h.push_back(0);
+  else h.insert(h.end(), tl.begin(), tl.end());
+  r.hi = h.size();
+  if (th.empty()) h.push_back(0);
+  else h.insert(h.end(), th.begin(), th.end());
+  r.P = (uint32_t)P;
+  r.single = hi.empty() ? 1u : 0u;
+  return r;
+}
+
+OffTab make_tab(const int64_t* base, const TabRef& r) {
+  OffTab t;
+  t.lo = base + r.lo;
+  t.hi = base + r.hi;
+  t.div = FastDiv(r.P);
+  t.base = 0;
+  t.single = r.single;
+  return t;
+}
+
+void copy_desc(Batch& b, const muscle_hip_tensor_t& t, std::vector<muscle_hip_tensor_t>& dst) {
+  b.modes.emplace_back(t.modes, t.modes + t.rank);
+  b.ext.emplace_back(t.extents, t.extents + t.rank);
+  b.str.emplace_back(t.strides, t.strides + t.rank);
+  dst.push_back(t);
+}
+
+}  // namespace
+
+int batch_tuning_class(int64_t m, int64_t n) { return tuning_class(m, n); }
+
+std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
+                                   const muscle_hip_tensor_t* bs, const int32_t* tiles) {
+  std::unique_ptr<Batch> B(new Batch());
+  B->h = &h;
+  if (n < 0) fail(MUSCLE_HIP_INVALID_ARGUMENT, "negative batch size");
+  if (n > 0 && (!cs || !as || !bs)) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null tensor arrays");
+  struct Lowered {
+    ContractionDesc d;
+    TabRef t[9];
+    int tile;
+    int64_t cost;
+  };
+  std::vector<Lowered> lw((size_t)n);
+  const int T3[9][2] = {{0, 0}, {2, 0}, {2, 1}, {1, 1}, {0, 2}, {1, 2}, {3, 0}, {3, 1}, {3, 2}};
+  for (int64_t i = 0; i < n; ++i) {
+    check_tensor(&cs[i], "C");
+    check_tensor(&as[i], "A");
+    check_tensor(&bs[i], "B");
+    if (i == 0) B->dtype = cs[0].dtype;
+    if (cs[i].dtype != B->dtype || as[i].dtype != B->dtype || bs[i].dtype != B->dtype)
+      fail(MUSCLE_HIP_UNSUPPORTED, "batched contractions need one element type for every operand");
+    Lowered& L = lw[(size_t)i];
+    L.d = lower_binary(cs[i], as[i], bs[i]);
+    const ContractionDesc& d = L.d;
+    const int64_t lim = int64_t(1) << 31;
+    if (d.M >= lim || d.N >= lim || d.K >= lim || d.B >= lim)
+      fail(MUSCLE_HIP_UNSUPPORTED, "batched pair with an index class of 2^31 or more elements");
+    const std::vector<Dim>* cls[4] = {&d.m, &d.n, &d.k, &d.batch};
+    for (int t = 0; t < 9; ++t) L.t[t] = add_table(B->htab, *cls[T3[t][0]], T3[t][1]);
+    int tile = tiles ? tiles[i] : MUSCLE_HIP_TILE_AUTO;
+    if (tile < 0 || tile >= MUSCLE_HIP_TILE_COUNT) {
+      const int tc = h.tune[tuning_class(d.M, d.N)];
+      tile = tc >= 0 ? tc : default_tile(d.M, d.N);
+    }
+    L.tile = tile;
+    L.cost = (d.K + 15) / 16;
+    copy_desc(*B, cs[i], B->cs);
+    copy_desc(*B, as[i], B->as);
+    copy_desc(*B, bs[i], B->bs);
+    const bool cp = is_complex(B->dtype);
+    B->flops += (cp ? 8.0 : 2.0) * (double)d.M * (double)d.N * (double)d.K * (double)d.B;
+    for (const muscle_hip_tensor_t* t : {&cs[i], &as[i], &bs[i]}) {
+      double e = 1;
+      for (int r = 0; r < t->rank; ++r) e *= (double)t->extents[r];
+      B->bytes += e * (double)dtype_size(t->dtype);
+    }
+  }
+  // point the stored descriptors at the owned copies
+  for (size_t i = 0; i < B->cs.size(); ++i) {
+    muscle_hip_tensor_t* ts[3] = {&B->cs[i], &B->as[i], &B->bs[i]};
+    for (int j = 0; j < 3; ++j) {
+      ts[j]->modes = B->modes[3 * i + j].data();
+      ts[j]->extents = B->ext[3 * i + j].data();
+      ts[j]->strides = B->str[3 * i + j].data();
+    }
+  }
+  // work list: pairs with the longest tiles (largest K) first
+  std::vector<int64_t> order((size_t)n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t x, int64_t y) { return lw[(size_t)x].cost > lw[(size_t)y].cost; });
+  bool used[MUSCLE_HIP_TILE_COUNT] = {false, false, false};
+  for (int64_t i : order) {
+    const Lowered& L = lw[(size_t)i];
+    if (L.d.empty) continue;
+    const TileShape& ts = kTileShapes[L.tile];
+    const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
+    const int64_t items = tm * tn * L.d.B;
+    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
+    for (int64_t t = 0; t < items; ++t) B->hitems.push_back(make_int2((int)i, (int)t));
+    used[L.tile] = true;
+  }
+  // device arena: pairs | items | counter | tables
+  const size_t o_pairs = 0, o_items = al(sizeof(BatchPair) * (size_t)std::max<int64_t>(n, 1));
+  const size_t o_counter = o_items + al(sizeof(int2) * std::max<size_t>(B->hitems.size(), 1));
+  const size_t o_tab = o_counter + 256;
+  const size_t total = o_tab + al(B->htab.size() * sizeof(int64_t));
+  if (hipMalloc(&B->dmem, total) != hipSuccess) {
+    (void)hipGetLastError();
+    B->dmem = nullptr;
+    fail(MUSCLE_HIP_OUT_OF_MEMORY, "batch descriptor allocation failed");
+  }
+  char* base = (char*)B->dmem;
+  B->dpairs = (BatchPair*)(base + o_pairs);
+  B->ditems = (int2*)(base + o_items);
+  B->dcounter = (int32_t*)(base + o_counter);
+  const int64_t* dtab = (const int64_t*)(base + o_tab);
+  B->hpairs.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const Lowered& L = lw[(size_t)i];
+    BatchPair& P = B->hpairs[(size_t)i];
+    std::memset(&P, 0, sizeof P);
+    const bool sw = L.d.swapped;
+    P.A = sw ? bs[i].data : as[i].data;
+    P.B = sw ? as[i].data : bs[i].data;
+    P.C = cs[i].data;
+    P.M = (int32_t)L.d.M;
+    P.N = (int32_t)L.d.N;
+    P.K = (int32_t)L.d.K;
+    P.NB = (int32_t)L.d.B;
+    const TileShape& ts = kTileShapes[L.tile];
+    P.tiles_m = (int32_t)((L.d.M + ts.bm - 1) / ts.bm);
+    P.tiles_n = (int32_t)((L.d.N + ts.bn - 1) / ts.bn);
+    P.tile = L.tile;
+    auto minst = [](const std::vector<Dim>& v, int w) {
+      int64_t b = INT64_MAX;
+      for (auto& x : v)
+        if (x.s[w] != 0) b = std::min(b, x.s[w] < 0 ? -x.s[w] : x.s[w]);
+      return b;
+    };
+    P.a_kmajor = minst(L.d.k, 0) < minst(L.d.m, 0);
+    P.b_kmajor = minst(L.d.k, 1) < minst(L.d.n, 1);
+    OffTab* tabs[9] = {&P.rowA, &P.colB, &P.kA, &P.kB, &P.rowC, &P.colC, &P.bA, &P.bB, &P.bC};
+    // tables were appended in T3 order: rowA, kA, kB, colB, rowC, colC, bA, bB, bC
+    const int map[9] = {0, 3, 1, 2, 4, 5, 6, 7, 8};
+    for (int t = 0; t < 9; ++t) *tabs[t] = make_tab(dtab, L.t[map[t]]);
+  }
+  MH_HIP_CHECK(hipMemcpy(base + o_pairs, B->hpairs.data(), sizeof(BatchPair) * (size_t)n, hipMemcpyHostToDevice));
+  if (!B->hitems.empty())
+    MH_HIP_CHECK(hipMemcpy(base + o_items, B->hitems.data(), sizeof(int2) * B->hitems.size(),
+                           hipMemcpyHostToDevice));
+  if (!B->htab.empty())
+    MH_HIP_CHECK(hipMemcpy(base + o_tab, B->htab.data(), B->htab.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  int cus = 0;
+  MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
+  B->workgroups = std::max(1, cus) * kBatchWgPerCu;
+  const bool dbl = is_double(B->dtype), cp = is_complex(B->dtype);
+  for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
+    if (used[t]) B->lds = std::max(B->lds, batch_lds_bytes(dbl, cp, t));
+  return B;
+}
+
+void batch_launch(Batch& b, hipStream_t s) {
+  BatchArgs a;
+  a.pairs = b.dpairs;
+  a.items = b.ditems;
+  a.counter = b.dcounter;
+  a.nitems = (int64_t)b.hitems.size();
+  a.complex3m = b.h->complex_mult == 3;
+  launch_batch(is_double(b.dtype), is_complex(b.dtype), a, b.workgroups, std::max<size_t>(b.lds, 16), s);
+}
+
+void batch_run(Batch& b, int mode) {
+  Handle& h = *b.h;
+  if (mode == 1) {
+    for (size_t i = 0; i < b.cs.size(); ++i) {
+      Plan* p = get_plan(h, 1, MUSCLE_HIP_PRECISION_NATIVE, 1e-12, &b.cs[i], &b.as[i], &b.bs[i]);
+      run_contraction(h, *p, b.cs[i], b.as[i], &b.bs[i]);
+    }
+    return;
+  }
+  if (b.hitems.empty()) return;
+  if (mode == 2) {
+    batch_launch(b, h.stream);
+    MH_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (!b.exec) {
+    // capture on a private stream (the null stream cannot be captured), then
+    // replay the graph on the handle's stream
+    if (!b.capture_stream) MH_HIP_CHECK(hipStreamCreateWithFlags(&b.capture_stream, hipStreamNonBlocking));
+    MH_HIP_CHECK(hipStreamBeginCapture(b.capture_stream, hipStreamCaptureModeThreadLocal));
+    batch_launch(b, b.capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b.capture_stream, &g);
+    if (e != hipSuccess) fail(MUSCLE_HIP_HIP_ERROR, std::string("graph capture: ") + hipGetErrorString(e));
+    b.graph = g;
+    MH_HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+  }
+  MH_HIP_CHECK(hipGraphLaunch(b.exec, h.stream));
+}
+
+// Times every tile shape on up to `samples` pairs of each size class.
+void batch_autotune(Batch& b, int samples, int reps) {
+  Handle& h = *b.h;
+  std::vector<std::vector<int64_t>> by_class(Handle::kTuneClasses);
+  for (size_t i = 0; i < b.hpairs.size(); ++i) {
+    const BatchPair& P = b.hpairs[i];
+    if (P.M * (int64_t)P.N * P.NB == 0) continue;
+    by_class[tuning_class(P.M, P.N)].push_back((int64_t)i);
+  }
+  hipEvent_t e0, e1;
+  MH_HIP_CHECK(hipEventCreate(&e0));
+  MH_HIP_CHECK(hipEventCreate(&e1));
+  for (int c = 0; c < Handle::kTuneClasses; ++c) {
+    if (by_class[c].empty()) continue;
+    std::vector<int64_t>& idx = by_class[c];
+    if ((int)idx.size() > samples) idx.resize((size_t)samples);
+    std::vector<muscle_hip_tensor_t> cs, as, bs;
+    for (int64_t i : idx) {
+      cs.push_back(b.cs[(size_t)i]);
+      as.push_back(b.as[(size_t)i]);
+      bs.push_back(b.bs[(size_t)i]);
+    }
+    float best = 1e30f;
+    int best_t = default_tile(b.hpairs[(size_t)idx[0]].M, b.hpairs[(size_t)idx[0]].N);
+    for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t) {
+      std::vector<int32_t> tiles(idx.size(), t);
+      std::unique_ptr<Batch> sub = build_batch(h, (int64_t)idx.size(), cs.data(), as.data(), bs.data(), tiles.data());
+      batch_run(*sub, 2);  // warm-up
+      MH_HIP_CHECK(hipEventRecord(e0, h.stream));
+      for (int r = 0; r < reps; ++r) batch_run(*sub, 2);
+      MH_HIP_CHECK(hipEventRecord(e1, h.stream));
+      MH_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      MH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_t = t;
+      }
+    }
+    h.tune[c] = best_t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+namespace {
+template <typename F>
+int guarded_b(F&& f) {
+  try {
+    f();
+    return MUSCLE_HIP_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return MUSCLE_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MUSCLE_HIP_INVALID_ARGUMENT;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int muscle_hip_batch_create(muscle_hip_handle_t handle, int64_t n, const muscle_hip_tensor_t* cs,
+                            const muscle_hip_tensor_t* as, const muscle_hip_tensor_t* bs, const int32_t* tiles,
+                            muscle_hip_batch_t* batch) {
+  return guarded_b([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h || !batch) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle or batch pointer");
+    std::lock_guard<std::mutex> lk(h->mu);
+    *batch = reinterpret_cast<muscle_hip_batch_t>(build_batch(*h, n, cs, as, bs, tiles).release());
+  });
+}
+
+int muscle_hip_batch_run(muscle_hip_handle_t handle, muscle_hip_batch_t batch, int mode) {
+  return guarded_b([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!h || !b) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle or batch");
+    if (b->h != h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "batch belongs to another handle");
+    if (mode < 0 || mode > 2) fail(MUSCLE_HIP_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
+    std::lock_guard<std::mutex> lk(h->mu);
+    batch_run(*b, mode);
+  });
+}
+
+int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t* info) {
+  return guarded_b([&] {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b || !info) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null batch or info");
+    info->pairs = (int64_t)b->cs.size();
+    info->items = (int64_t)b->hitems.size();
+    info->workgroups = b->workgroups;
+    info->graph = b->exec ? 1 : 0;
+    info->flops = b->flops;
+    info->bytes = b->bytes;
+    info->table_bytes = (int64_t)(b->htab.size() * 8 + b->hpairs.size() * sizeof(BatchPair) +
+                                  b->hitems.size() * sizeof(int2));
+  });
+}
+
+int muscle_hip_batch_destroy(muscle_hip_batch_t batch) {
+  return guarded_b([&] { delete reinterpret_cast<Batch*>(batch); });
+}
+
+int muscle_hip_batch_autotune(muscle_hip_handle_t handle, muscle_hip_batch_t batch, int samples, int reps) {
+  return guarded_b([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!h || !b || b->h != h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null or foreign batch");
+    std::lock_guard<std::mutex> lk(h->mu);
+    batch_autotune(*b, std::max(1, samples), std::max(1, reps));
+  });
+}
+
+int muscle_hip_tuning_save(muscle_hip_handle_t handle, const char* path) {
+  return guarded_b([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h || !path) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle or path");
+    std::ofstream f(path);
+    if (!f) fail(MUSCLE_HIP_INVALID_ARGUMENT, std::string("cannot write ") + path);
+    f << "# muscle_hip batched tile tuning: class tile (class = 5 * bucket(M) + bucket(N))\n";
+    for (int c = 0; c < Handle::kTuneClasses; ++c)
+      if (h->tune[c] >= 0) f << c << " " << h->tune[c] << "\n";
+  });
+}
+
+int muscle_hip_tuning_load(muscle_hip_handle_t handle, const char* path) {
+  return guarded_b([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h || !path) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle or path");
+    std::ifstream f(path);
+    if (!f) fail(MUSCLE_HIP_INVALID_ARGUMENT, std::string("cannot read ") + path);
+    std::string line;
+    while (std::getline(f, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      int c = -1, t = -1;
+      if (std::sscanf(line.c_str(), "%d %d", &c, &t) == 2 && c >= 0 && c < Handle::kTuneClasses && t >= 0 &&
+          t < MUSCLE_HIP_TILE_COUNT)
+        h->tune[c] = t;
+    }
+  });
+}
+
+int muscle_hip_tuning_class(int64_t m, int64_t n) { return batch_tuning_class(m, n); }
+
+}  // extern "C"

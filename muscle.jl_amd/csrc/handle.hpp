@@ -83,6 +83,9 @@ struct Handle {
   void* ws = nullptr;
   size_t ws_size = 0;
   int complex_mult = 3;  // MUSCLE_HIP_OPTION_COMPLEX_MULT
+  static constexpr int kTuneClasses = 25;
+  int tune[kTuneClasses] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                            -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};  // batched tile per size class
 
   ~Handle();
   void ensure_workspace(size_t bytes);

@@ -42,6 +42,21 @@ class TensorDesc(ctypes.Structure):
     ]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_int64),
+        ("items", ctypes.c_int64),
+        ("workgroups", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("flops", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+        ("table_bytes", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Info(ctypes.Structure):
     _fields_ = [
         ("kernel", ctypes.c_int32),
@@ -84,6 +99,15 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_binary_einsum": ([H, T, T, T, ctypes.c_int, ctypes.c_double, I], ctypes.c_int),
         "muscle_hip_hadamard": ([H, T, T, T, I], ctypes.c_int),
         "muscle_hip_unary_einsum": ([H, T, T, I], ctypes.c_int),
+        "muscle_hip_batch_create": ([H, ctypes.c_int64, T, T, T, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "muscle_hip_batch_run": ([H, ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+        "muscle_hip_batch_get_info": ([ctypes.c_void_p, ctypes.POINTER(BatchInfo)], ctypes.c_int),
+        "muscle_hip_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
+        "muscle_hip_batch_autotune": ([H, ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "muscle_hip_tuning_save": ([H, ctypes.c_char_p], ctypes.c_int),
+        "muscle_hip_tuning_load": ([H, ctypes.c_char_p], ctypes.c_int),
+        "muscle_hip_tuning_class": ([ctypes.c_int64, ctypes.c_int64], ctypes.c_int),
         "muscle_hip_describe": ([T, T, T, ctypes.c_int, ctypes.c_double, I, ctypes.POINTER(ctypes.c_int32),
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     }

@@ -101,3 +101,39 @@ def ladder_inputs(dist: str, seed: int = 0):
     else:
         raise ValueError(dist)
     return A, ["i", "j", "k"], B, ["k", "l", "j"], ["j", "k"], ["i", "l"]
+
+
+# ------------------------------------------------------------ batched uneven pairs
+def circuit_pairs(n: int = 10000, seed: int = 0, ranks=(4, 22), contracted=(1, 10), hyper=(0, 2),
+                  out_ranks=(4, 24)):
+    """Index structure of n pairwise contractions of a circuit-like network:
+    every index has dimension 2; tensor ranks uniform in `ranks`; 1-10
+    contracted and 0-2 hyper (batch) indices per pair; output ranks kept in
+    `out_ranks` (element counts 2^4..2^24).  Index orders are random
+    permutations, as contraction trees produce them.  Returns a list of
+    (inds_a, inds_b, inds_c) with labels unique within a pair."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        ra, rb = (int(x) for x in rng.integers(ranks[0], ranks[1] + 1, 2))
+        c = int(rng.integers(contracted[0], contracted[1] + 1))
+        h = int(rng.integers(hyper[0], hyper[1] + 1))
+        L, R = ra - c - h, rb - c - h
+        if L < 0 or R < 0 or not (out_ranks[0] <= L + R + h <= out_ranks[1]):
+            continue
+        lab = iter(range(10 ** 6))
+        left = [f"l{next(lab)}" for _ in range(L)]
+        right = [f"r{next(lab)}" for _ in range(R)]
+        con = [f"c{next(lab)}" for _ in range(c)]
+        hyp = [f"h{next(lab)}" for _ in range(h)]
+        ia = [x for x in rng.permutation(left + con + hyp)]
+        ib = [x for x in rng.permutation(con + hyp + right)]
+        ic = [x for x in rng.permutation(left + right + hyp)]
+        out.append((ia, ib, ic))
+    return out
+
+
+def pair_flops(pair, complex_=True) -> float:
+    ia, ib, ic = pair
+    labels = set(ia) | set(ib)
+    return (8.0 if complex_ else 2.0) * 2.0 ** len(labels)

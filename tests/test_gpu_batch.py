@@ -1,0 +1,125 @@
+"""Batched contractions (one persistent, graph-captured launch) against the
+CPU oracle, pair by pair: norm-wise relative error <= 1e-12 for ComplexF64 /
+Float64 and <= 2e-5 for ComplexF32.  Graph replays are compared with the
+eager launch after host work in between (every replay is checked)."""
+import numpy as np
+import pytest
+import torch
+
+import muscle
+from muscle.batch import ContractionBatch, tuning_class, tuning_load, tuning_save
+from muscle.workloads import circuit_pairs
+from oracle import backend_base as bb
+from tests.gpu_util import dev_tensor, host, randn, rel_err
+
+pytestmark = pytest.mark.gpu
+
+GENERAL = [
+    (["i", "j", "k"], (64, 32, 48), ["k", "l", "j"], (48, 96, 32), ["i", "l"]),
+    (["i", "j"], (67, 45), ["j", "k"], (45, 93), ["k", "i"]),
+    (["i", "j", "h"], (40, 36, 3), ["j", "k", "h"], (36, 50, 3), ["h", "k", "i"]),
+    (["i", "j"], (300, 20), ["j"], (20,), ["i"]),
+    (["j"], (200,), ["j", "k"], (200, 300), ["k"]),
+    (["i", "j"], (120, 130), ["j", "i"], (130, 120), []),
+    (["i", "j", "s"], (30, 40, 5), ["j", "k"], (40, 35), ["i", "k"]),
+    (["i", "j", "i"], (20, 30, 20), ["j", "k"], (30, 40), ["i", "k"]),
+    (["a", "b"], (5, 7), ["c"], (9,), ["c", "a", "b"]),
+]
+
+
+def make_pairs(rng, T, n_circuit=120, max_rank=11, seed=3):
+    cases = []
+    for ia, ib, ic in circuit_pairs(n_circuit, seed=seed, ranks=(4, max_rank), out_ranks=(0, 14)):
+        cases.append((ia, (2,) * len(ia), ib, (2,) * len(ib), ic))
+    cases += GENERAL
+    pairs, refs = [], []
+    for ia, sa, ib, sb, ic in cases:
+        A, B = randn(rng, sa, T), randn(rng, sb, T)
+        a, b = dev_tensor(A, ia), dev_tensor(B, ib)
+        size = dict(zip(ia, sa)) | dict(zip(ib, sb))
+        c = muscle.Tensor(muscle.empty_col_major([size[x] for x in ic], a.dtype, a.device), ic)
+        pairs.append((c, a, b))
+        refs.append(bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib))
+    return pairs, refs
+
+
+def check(pairs, refs, tol):
+    torch.cuda.synchronize()
+    worst = max(rel_err(host(c), r) for (c, a, b), r in zip(pairs, refs))
+    assert worst <= tol, worst
+
+
+@pytest.mark.parametrize("T, tol", [(np.complex128, 1e-12), (np.float64, 1e-12), (np.complex64, 2e-5)])
+def test_batch_matches_oracle(T, tol):
+    rng = np.random.default_rng(11)
+    pairs, refs = make_pairs(rng, T)
+    batch = ContractionBatch(pairs)
+    info = batch.info()
+    assert info["pairs"] == len(pairs) and info["items"] > 0
+    batch.run("persistent")
+    check(pairs, refs, tol)
+    assert batch.info()["graph"] == 1
+    for c, a, b in pairs:
+        c.parent().zero_()
+    batch.run("loop")
+    check(pairs, refs, tol)
+    batch.close()
+
+
+def test_graph_replays_equal_eager():
+    rng = np.random.default_rng(12)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=60)
+    batch = ContractionBatch(pairs)
+    batch.run("persistent_nograph")
+    torch.cuda.synchronize()
+    eager = [c.parent().clone() for c, a, b in pairs]
+    for rep in range(3):
+        for c, a, b in pairs:
+            c.parent().fill_(float("nan"))
+        _ = torch.randn(1000, device="cuda").sum().item()  # unrelated eager work between replays
+        batch.run("persistent")
+        torch.cuda.synchronize()
+        for (c, a, b), e in zip(pairs, eager):
+            assert torch.equal(c.parent(), e), rep
+    check(pairs, refs, 1e-12)
+
+
+@pytest.mark.parametrize("tile", ["64x64", "32x128", "128x32"])
+def test_every_tile_shape(tile):
+    rng = np.random.default_rng(13)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=60)
+    batch = ContractionBatch(pairs, tiles=[tile] * len(pairs))
+    batch.run()
+    check(pairs, refs, 1e-12)
+
+
+@pytest.mark.parametrize("mult", [3, 4])
+def test_complex_products(mult):
+    rng = np.random.default_rng(14)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=40)
+    ContractionBatch(pairs, complex_mult=mult).run()
+    check(pairs, refs, 1e-12)
+
+
+def test_autotune_and_tuning_file(tmp_path):
+    rng = np.random.default_rng(15)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=80)
+    batch = ContractionBatch(pairs)
+    batch.autotune(samples=8, reps=2)
+    path = str(tmp_path / "tune.txt")
+    tuning_save(path)
+    lines = [l for l in open(path) if not l.startswith("#")]
+    assert lines, "autotuner recorded no size class"
+    tuning_load(path)
+    tuned = ContractionBatch(pairs)  # AUTO tiles now come from the tuning table
+    tuned.run()
+    check(pairs, refs, 1e-12)
+    assert 0 <= tuning_class(1, 1) < 25 and tuning_class(1000, 1000) == 24
+
+
+def test_batch_rejects_mixed_eltypes():
+    a = dev_tensor(np.ones((4, 4)), ["i", "j"])
+    b = dev_tensor(np.ones((4, 4), dtype=np.complex128), ["j", "k"])
+    c = muscle.Tensor(torch.zeros((4, 4), dtype=torch.complex128, device="cuda"), ["i", "k"])
+    with pytest.raises(muscle.UnsupportedError):
+        ContractionBatch([(c, a, b)])
