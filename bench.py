@@ -1,12 +1,25 @@
-"""Benchmark of the contraction hot path on one MI355X.
+"""Benchmark of Muscle.jl's contraction hot path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload NAME] [--precision P]
+    python bench.py [--steps K] [--warmup W] [--only NAMES]
 
-A step is one binary_einsum call over the workload's inputs, which are
-resident in HBM before the timed region starts.  Default workload: the
-large GEMM-shaped contraction on permuted layouts, ComplexF64, rank-6
-tensors with every extent 16, contracted modes at positions (2,4,6) of A and
-(1,3,5) of B, output order (a1,b1,a3,b2,a5,b3): M = N = K = 4096.
+A step is one binary_einsum call over the headline workload's inputs, which
+are resident in HBM before the timed region starts.  Headline: the large
+GEMM-shaped contraction on permuted layouts, ComplexF64, rank-6 tensors with
+every extent 16, contracted modes at positions (2,4,6) of A and (1,3,5) of B,
+output order (a1,b1,a3,b2,a5,b3): M = N = K = 4096, plain FP64 arithmetic.
+
+Beside it, in "extra":
+  ladder_c2   the same contraction on every precision rung (FP32, BF16x3,
+              FP16x2, Ozaki int8 at 1e-10 and 1e-12) with its relative error
+              against the CPU oracle's result and its speed-up over plain;
+  c2_c32, c2h_c64, c1_c64  ComplexF32, hyperindex (batch 64 x 512^3) and
+              the small plain case (latency-bound, reported in us);
+  ladder_c1   config-1 shapes under three value distributions, error of each
+              rung against the oracle;
+  batched_c64 10,000 uneven dims-2 pairwise contractions as persistent
+              graph-replayed launches vs a loop of plain calls (pairs/s);
+  served      contraction-tree jobs through the continuous-batching server
+              (jobs/s, p50/p99 latency, error of final tensors).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -23,130 +36,348 @@ for _p in (ROOT, os.path.join(ROOT, "muscle.jl_amd")):
         sys.path.insert(0, _p)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
-HBM_GBS = 8000.0
-
-WORKLOADS = {
-    # name: (description, element type)
-    "c2_c64": ("config2 rank-6 interleaved, every extent 16, M=N=K=4096, ComplexF64", "c64"),
-    "c2_c32": ("config2 rank-6 interleaved, every extent 16, M=N=K=4096, ComplexF32", "c32"),
-    "c2h_c64": ("config2 hyperindex: batch 64 x (M=N=K=512), ComplexF64", "c64"),
-    "c1_c64": ("config1 plain: A(64,32,48) B(48,96,32) -> C(64,96), ComplexF64", "c64"),
-}
+HBM_TBS = 8.0
+METRIC = "binary_einsum TFLOP/s (% MFMA peak) vs plain at stated error; jobs/s + p99 latency"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_inputs(name: str):
+def rel(got, ref) -> float:
     import numpy as np
-
-    from muscle import workloads as wl
-    if name in ("c2_c64", "c2_c32"):
-        dt = np.complex128 if name == "c2_c64" else np.complex64
-        A, ia, B, ib, ic = wl.config2_inputs(16, seed=0, dtype=dt)
-        return A, ia, B, ib, {"out": ic}
-    if name == "c2h_c64":
-        A, ia, B, ib, ic = wl.config2h_inputs(1, seed=0)
-        return A, ia, B, ib, {"out": ic}
-    if name == "c1_c64":
-        A, ia, B, ib, dims, ic = wl.config1_inputs(0)
-        return A, ia, B, ib, {"dims": dims}
-    raise SystemExit(f"unknown workload {name}")
+    got = np.asarray(got, dtype=np.complex128)
+    ref = np.asarray(ref, dtype=np.complex128)
+    return float(np.linalg.norm((got - ref).ravel()) / np.linalg.norm(ref.ravel()))
 
 
-def cpu_baseline(A, ia, B, ib, kw, threads: int, reps: int):
+def put(x, inds, dev):
+    import numpy as np
+    import torch
+
+    import muscle
+    return muscle.Tensor(torch.from_numpy(np.asfortranarray(x)).to(dev), inds)
+
+
+def timed(fn, steps: int, warmup: int):
+    """Mean ms per call on the current stream (HIP events bracket the calls)."""
+    import torch
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        out = fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, out
+
+
+def contraction_case(a, b, kw, backend, steps, warmup):
+    import muscle
+
+    def step():
+        with muscle.with_backend(backend):
+            return muscle.binary_einsum(a, b, **kw)
+    ms, c = timed(step, steps, warmup)
+    return ms, c, muscle.last_info()
+
+
+# ------------------------------------------------------------------ CPU baseline
+def cpu_oracle(A, ia, B, ib, inds_c, threads: int, reps: int):
     """The oracle's BackendBase restatement (permute + BLAS gemm + permute) on the host cores."""
     import numpy as np
     from threadpoolctl import threadpool_info, threadpool_limits
 
     from oracle import backend_base as bb
-    inds_c = bb.front_binary(ia, ib, dims=kw.get("dims"), out=kw.get("out"))
-    f = bb.binary_einsum_base if not any(i in ia and i in ib for i in inds_c) else bb.binary_einsum_general
+    general = any(i in ia and i in ib for i in inds_c)
+    f = bb.binary_einsum_general if general else bb.binary_einsum_base
     times = []
     with threadpool_limits(limits=threads):
-        f(inds_c, A, ia, B, ib)  # warm-up
+        ref = f(inds_c, A, ia, B, ib)
         for _ in range(reps):
             t0 = time.perf_counter()
             f(inds_c, A, ia, B, ib)
             times.append(time.perf_counter() - t0)
     blas = [f"{d.get('internal_api')} {d.get('version')}" for d in threadpool_info() if d.get("user_api") == "blas"]
-    return float(np.median(times)), blas
+    return float(np.median(times)), ref, blas
 
 
+# ------------------------------------------------------------------ batched (config 4)
+def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, loop_steps: int, cpu_sample: int,
+                  threads: int):
+    import numpy as np
+    import torch
+
+    import muscle
+    from muscle.batch import ContractionBatch
+    from muscle.workloads import circuit_pairs
+    dev = torch.device("cuda", 0)
+    tdt = torch.complex128 if dtype == "c64" else torch.complex64
+    esz = 16 if dtype == "c64" else 8
+    t0 = time.perf_counter()
+    pairs = circuit_pairs(n, seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pool_n = 1 << 23  # inputs are views into one random pool (read-only, as tree tensors are reused)
+    pool = torch.randn(pool_n, dtype=tdt, device=dev, generator=g)
+    rng = np.random.default_rng(seed + 1)
+    budget = int(budget_gb * 2 ** 30) // esz
+    rounds, cur, used = [], [], 0
+    for p in pairs:
+        sz = 1 << len(p[2])
+        if cur and used + sz > budget:
+            rounds.append(cur)
+            cur, used = [], 0
+        cur.append(p)
+        used += sz
+    rounds.append(cur)
+    arena_n = max(sum(1 << len(p[2]) for p in r) for r in rounds)
+    arena = torch.empty(arena_n, dtype=tdt, device=dev)
+
+    def dims2(base, rank, off):
+        return base.as_strided((2,) * rank, tuple(1 << i for i in range(rank)), off)
+
+    round_tensors = []
+    for r in rounds:
+        ts, off = [], 0
+        for ia, ib, ic in r:
+            a = muscle.Tensor(dims2(pool, len(ia), int(rng.integers(0, pool_n - (1 << len(ia)) + 1))), ia)
+            b = muscle.Tensor(dims2(pool, len(ib), int(rng.integers(0, pool_n - (1 << len(ib)) + 1))), ib)
+            c = muscle.Tensor(dims2(arena, len(ic), off), ic)
+            off += 1 << len(ic)
+            ts.append((c, a, b))
+        round_tensors.append(ts)
+    log(f"batched {dtype}: {n} pairs in {len(rounds)} round(s) of <= {budget_gb:g} GiB outputs "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+    probe = ContractionBatch(round_tensors[0])
+    t1 = time.perf_counter()
+    probe.autotune(samples=16, reps=2)
+    tune_s = time.perf_counter() - t1
+    probe.close()
+    batches = [ContractionBatch(ts) for ts in round_tensors]
+    infos = [b.info() for b in batches]
+    flops = sum(i["flops"] for i in infos)
+    nbytes = sum(i["bytes"] for i in infos)
+    # per-pair roofline: max(flops / MFMA peak, bytes / HBM bandwidth), summed
+    peak = (PEAK_TFLOPS["fp64"] if dtype == "c64" else PEAK_TFLOPS["fp32"]) * 1e12
+    roof_s = 0.0
+    for ia, ib, ic in pairs:
+        fl = 8.0 * 2.0 ** len(set(ia) | set(ib))
+        by = esz * (2.0 ** len(ia) + 2.0 ** len(ib) + 2.0 ** len(ic))
+        roof_s += max(fl / peak, by / (HBM_TBS * 1e12))
+
+    def run_all(mode):
+        for bt in batches:
+            bt.run(mode)
+    ms_p, _ = timed(lambda: run_all("persistent"), steps, 1)
+    last_c = [c.parent().clone() for c, a, b in round_tensors[-1]]
+    ms_l, _ = timed(lambda: run_all("loop"), loop_steps, 1)
+    diff = max(float(torch.linalg.vector_norm(c.parent() - x) / torch.linalg.vector_norm(x).clamp_min(1e-300))
+               for (c, a, b), x in zip(round_tensors[-1], last_c))
+    out = {
+        "dtype": dtype, "pairs": n, "rounds": len(rounds), "output_budget_gib": budget_gb,
+        "items": sum(i["items"] for i in infos), "workgroups": infos[0]["workgroups"],
+        "graph": all(bt.info()["graph"] for bt in batches),
+        "persistent_ms": round(ms_p, 3), "persistent_pairs_per_s": round(n / (ms_p / 1e3), 1),
+        "persistent_tflops": round(flops / (ms_p / 1e3) / 1e12, 3),
+        "loop_ms": round(ms_l, 3), "loop_pairs_per_s": round(n / (ms_l / 1e3), 1),
+        "speedup_vs_plain_loop": round(ms_l / ms_p, 3),
+        "roofline_ms": round(roof_s * 1e3, 3), "roofline_frac": round(roof_s * 1e3 / ms_p, 4),
+        "total_tflop": round(flops / 1e12, 3), "total_gb": round(nbytes / 1e9, 2),
+        "autotune_s": round(tune_s, 2), "persistent_vs_loop_max_rel_diff": diff,
+    }
+    if cpu_sample:
+        # CPU baseline on a bounded sample: the oracle restatement on the first pairs
+        from threadpoolctl import threadpool_limits
+
+        from oracle import backend_base as bb
+        sample = pairs[:cpu_sample]
+        rs = np.random.default_rng(seed + 2)
+        tc = 0.0
+        with threadpool_limits(limits=threads):
+            for ia, ib, ic in sample:
+                A = (rs.standard_normal((2,) * len(ia)) + 1j * rs.standard_normal((2,) * len(ia)))
+                B = (rs.standard_normal((2,) * len(ib)) + 1j * rs.standard_normal((2,) * len(ib)))
+                t = time.perf_counter()
+                bb.binary_einsum_general(ic, A, ia, B, ib)
+                tc += time.perf_counter() - t
+        out["cpu_sample_pairs"] = cpu_sample
+        out["cpu_pairs_per_s"] = round(cpu_sample / tc, 1)
+    for bt in batches:
+        bt.close()
+    del arena, pool
+    torch.cuda.empty_cache()
+    return out
+
+
+def measured_traffic(precision: str):
+    """HBM bytes per launch of the headline kernel, from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (profiles/traffic_config2_c64.txt), or None."""
+    if precision != "native":
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic_config2_c64.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        return None
+    return None
+
+
+# ------------------------------------------------------------------ main
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2_c64", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="native")
+    ap.add_argument("--only", default="all",
+                    help="comma list of: ladder_c2,c2_c32,c2h_c64,c1_c64,ladder_c1,batched,served "
+                         "(the headline always runs)")
+    ap.add_argument("--precision", default="native", help="arithmetic of the headline workload")
+    ap.add_argument("--target", type=float, default=1e-10)
     ap.add_argument("--complex-mult", type=int, default=3, choices=[3, 4])
-    ap.add_argument("--target", type=float, default=1e-10, help="target relative error of the Ozaki rung")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-pairs", type=int, default=10000)
+    ap.add_argument("--batch-budget-gib", type=float, default=96.0)
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="after the headline, copy this many GiB device-to-device 3 times (a kernel of known "
+                         "traffic to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only: the contraction is sharded by output "
-                                   "blocks across GPUs with no collective (see DESIGN.md), and no multi-GPU "
-                                   "figure is produced"}))
+        print(json.dumps({"error": "this build measures one GPU only; how the path shards over GPUs is "
+                                   "described in DESIGN.md, and no multi-GPU figure is produced"}))
         return 2
+    only = set(args.only.split(","))
+
+    def want(k):
+        return "all" in only or k in only
 
     import numpy as np
     import torch
 
     import muscle
-
+    from muscle import workloads as wl
     dev = torch.device("cuda", 0)
-    log(f"workload {args.workload}: generating inputs")
-    A, ia, B, ib, kw = make_inputs(args.workload)
-    a = muscle.Tensor(torch.from_numpy(np.asfortranarray(A)).to(dev), ia)
-    b = muscle.Tensor(torch.from_numpy(np.asfortranarray(B)).to(dev), ib)
-    backend = muscle.BackendHIP(precision=args.precision, complex_mult=args.complex_mult,
-                                target_rel_error=args.target)
+    threads = int(os.environ.get("MUSCLE_BENCH_CPU_THREADS", "16"))
+    extra = {}
 
-    def step():
-        with muscle.with_backend(backend):
-            return muscle.binary_einsum(a, b, **kw)
-
-    log(f"warmup {args.warmup}")
-    c = None
-    for _ in range(args.warmup):
-        c = step()
-    torch.cuda.synchronize()
-    info = muscle.last_info()
+    # ---------------- headline: config 2, ComplexF64
+    log("headline: config2 ComplexF64 (M=N=K=4096, interleaved rank-6 layouts)")
+    A, ia, B, ib, ic = wl.config2_inputs(16, seed=0, dtype=np.complex128)
+    a, b = put(A, ia, dev), put(B, ib, dev)
+    plain = muscle.BackendHIP(complex_mult=args.complex_mult)
+    head_be = muscle.BackendHIP(precision=args.precision, target_rel_error=args.target,
+                                complex_mult=args.complex_mult)
+    ms, c, info = contraction_case(a, b, {"out": ic}, head_be, args.steps, args.warmup)
     flops = info["flops"]
-    stream = torch.cuda.current_stream(dev)  # the library enqueues on torch's current stream
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        c = step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
     tflops = flops / (ms * 1e-3) / 1e12
-    complex32 = a.dtype in (torch.complex64, torch.float32) and b.dtype in (torch.complex64, torch.float32)
-    peak = PEAK_TFLOPS["fp32" if complex32 and args.precision == "native" else "fp64"]
-    log(f"{ms:.3f} ms/step, {tflops:.2f} TFLOP/s (wall {wall:.3f} s)")
-
-    # parity spot-check of the timed result (sampled entries, oracle dot products)
-    del c
-
-    cpu = None
+    log(f"headline {ms:.3f} ms/step, {tflops:.2f} TFLOP/s")
+    cpu, ref_c2 = None, None
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("MUSCLE_BENCH_CPU_THREADS", "16"))
         log(f"cpu baseline: oracle BackendBase restatement, {threads} threads, median of {args.cpu_reps}")
-        t_cpu, blas = cpu_baseline(A, ia, B, ib, kw, threads, args.cpu_reps)
-        cpu = {"value": flops / t_cpu / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-               "sample": f"the full {args.workload} contraction, median of {args.cpu_reps} runs "
-                         f"({t_cpu:.3f} s each; numpy {np.__version__}, BLAS {'; '.join(blas)})"}
+        t_cpu, ref_c2, blas = cpu_oracle(A, ia, B, ib, ic, threads, args.cpu_reps)
+        cpu = {"value": round(flops / t_cpu / 1e12, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+               "sample": f"the full config-2 ComplexF64 contraction (M=N=K=4096), median of {args.cpu_reps} runs "
+                         f"of {t_cpu:.3f} s; numpy {np.__version__}, BLAS {'; '.join(blas)}"}
+        extra["headline_rel_err_vs_cpu_oracle"] = rel(c.parent().cpu().numpy(), ref_c2)
+    plain_ms = ms
+    del c
+    if args.copy_calibration > 0:
+        nel = int(args.copy_calibration * 2 ** 30) // 16
+        src = torch.ones(nel, dtype=torch.complex128, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        extra["copy_calibration_bytes_each_way"] = nel * 16
+        del src, dst
 
+    # ---------------- precision ladder on config 2
+    if want("ladder_c2") and args.precision == "native":
+        lad = {}
+        for mode, tgt in [("fp32", 0), ("bf16x3", 0), ("fp16x2", 0), ("ozaki_int8", 1e-10), ("ozaki_int8", 1e-12)]:
+            be = muscle.BackendHIP(precision=mode, target_rel_error=tgt or 1e-12, complex_mult=args.complex_mult)
+            m2, c2, inf2 = contraction_case(a, b, {"out": ic}, be, max(3, args.steps // 2), 1)
+            key = mode if mode != "ozaki_int8" else f"ozaki_int8_{tgt:g}"
+            lad[key] = {"ms": round(m2, 4), "tflops": round(flops / (m2 * 1e-3) / 1e12, 3),
+                        "speedup_vs_plain": round(plain_ms / m2, 3), "slices": inf2["slices"],
+                        "err_model": inf2["err_bound"]}
+            if ref_c2 is not None:
+                lad[key]["rel_err_vs_cpu_oracle"] = rel(c2.parent().cpu().numpy(), ref_c2)
+            log(f"ladder {key}: {lad[key]}")
+            del c2
+        m4, c4m, _ = contraction_case(a, b, {"out": ic}, muscle.BackendHIP(complex_mult=4), max(3, args.steps // 2),
+                                      1)
+        lad["plain_4m"] = {"ms": round(m4, 4), "tflops": round(flops / (m4 * 1e-3) / 1e12, 3)}
+        if ref_c2 is not None:
+            lad["plain_4m"]["rel_err_vs_cpu_oracle"] = rel(c4m.parent().cpu().numpy(), ref_c2)
+        del c4m
+        extra["ladder_c2"] = lad
+    del a, b
+    torch.cuda.empty_cache()
+
+    # ---------------- other configurations
+    if want("c2_c32"):
+        A32, ia2, B32, ib2, ic2 = wl.config2_inputs(16, seed=0, dtype=np.complex64)
+        m2, c2, _ = contraction_case(put(A32, ia2, dev), put(B32, ib2, dev), {"out": ic2}, plain, args.steps, 2)
+        extra["c2_c32"] = {"ms": round(m2, 4), "tflops": round(flops / (m2 * 1e-3) / 1e12, 3),
+                           "frac_fp32_peak": round(flops / (m2 * 1e-3) / 1e12 / PEAK_TFLOPS["fp32"], 4)}
+        del c2
+    if want("c2h_c64"):
+        Ah, iah, Bh, ibh, ich = wl.config2h_inputs(1, seed=0)
+        m2, c2, inf2 = contraction_case(put(Ah, iah, dev), put(Bh, ibh, dev), {"out": ich}, plain, args.steps, 2)
+        extra["c2h_c64"] = {"ms": round(m2, 4), "tflops": round(inf2["flops"] / (m2 * 1e-3) / 1e12, 3),
+                            "batch": inf2["batch"], "m": inf2["m"], "n": inf2["n"], "k": inf2["k"]}
+        del c2
+    if want("c1_c64"):
+        A1, ia1, B1, ib1, dims, ic1 = wl.config1_inputs(0)
+        a1, b1 = put(A1, ia1, dev), put(B1, ib1, dev)
+        m1, c1, inf1 = contraction_case(a1, b1, {"dims": dims}, plain, 200, 10)
+        extra["c1_c64"] = {"us_per_call": round(m1 * 1e3, 2), "tflops": round(inf1["flops"] / (m1 * 1e-3) / 1e12, 4)}
+    if want("ladder_c1") and not args.no_cpu_baseline:
+        from oracle import backend_base as bb
+        res = {}
+        for dist in ["normal", "loguniform", "unitrows"]:
+            A1, ia1, B1, ib1, dims, ic1 = wl.ladder_inputs(dist, 0)
+            ref = bb.binary_einsum_base(ic1, A1, ia1, B1, ib1)
+            a1, b1 = put(A1, ia1, dev), put(B1, ib1, dev)
+            row = {}
+            for mode, tgt in [("native", 0), ("fp32", 0), ("bf16x3", 0), ("fp16x2", 0), ("ozaki_int8", 1e-10),
+                              ("ozaki_int8", 1e-12)]:
+                be = muscle.BackendHIP(precision=mode, target_rel_error=tgt or 1e-12)
+                m1, c1, _ = contraction_case(a1, b1, {"dims": dims}, be, 20, 2)
+                got = c1.parent().cpu().numpy().astype(np.complex128)
+                el = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+                key = mode if mode != "ozaki_int8" else f"ozaki_int8_{tgt:g}"
+                row[key] = {"normwise": rel(got, ref), "max_elementwise": float(el.max()),
+                            "mean_elementwise": float(el.mean()), "us_per_call": round(m1 * 1e3, 2)}
+            res[dist] = row
+        extra["ladder_c1"] = res
+    if want("batched"):
+        try:
+            extra["batched_c64"] = bench_batched("c64", args.batch_pairs, 0, args.batch_budget_gib, 3, 1,
+                                                 0 if args.no_cpu_baseline else 300, threads)
+            log(f"batched: {extra['batched_c64']}")
+        except Exception as e:  # reported in the JSON, not hidden
+            extra["batched_c64"] = {"error": repr(e)}
+            log(f"batched failed: {e!r}")
+    if want("served"):
+        try:
+            from muscle.serve import bench_served
+        except ImportError:
+            bench_served = None
+        if bench_served is not None:
+            extra["served"] = bench_served(seed=0, log=log)
+
+    peak = PEAK_TFLOPS["fp64"]
     out = {
-        "metric": "binary_einsum TFLOP/s (% MFMA peak) vs plain at stated error; jobs/s + p99 latency",
+        "metric": METRIC,
         "value": round(tflops, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -156,16 +387,16 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"c64": "complex128 (fp64 mfma)", "c32": "complex64 (fp32 mfma)"}[WORKLOADS[args.workload][1]]
-        if args.precision == "native" else f"complex128 via {args.precision}",
+        "dtype": "complex128 (fp64 mfma, 3M complex products)" if args.precision == "native"
+        else f"complex128 via {args.precision}",
         "data": "synthetic (seeded numpy CN(0,1)/sqrt(K))",
-        "config": {"workload": WORKLOADS[args.workload][0], "precision": args.precision,
-                   "complex_mult": args.complex_mult,
-                   "m": info["m"], "n": info["n"], "k": info["k"], "batch": info["batch"],
-                   "kernel": info["kernel"], "flops_per_step": flops},
+        "config": {"workload": "config2: rank-6 interleaved layouts, every extent 16, M=N=K=4096, ComplexF64",
+                   "precision": args.precision, "complex_mult": args.complex_mult, "m": info["m"], "n": info["n"],
+                   "k": info["k"], "batch": info["batch"], "kernel": info["kernel"], "flops_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(tflops / peak, 4), "traffic": None},
+                     "frac": round(tflops / peak, 4), "traffic": measured_traffic(args.precision)},
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(out), flush=True)
     return 0
