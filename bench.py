@@ -214,6 +214,133 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
     return out
 
 
+def served_run(prepared, mode, rate, pool_bytes, seed, keep=()):
+    """Submits prepared jobs at Poisson arrival times (rate jobs/s; rate 0: all
+    at once) to a fresh server and collects per-job latency from the arrival
+    time to completion.  Returns (summary, {job index: root tensor})."""
+    import threading
+
+    import numpy as np
+
+    from muscle.serve import Server
+    srv = Server(pool_gib=pool_bytes / 2 ** 30, mode=mode)
+    rng = np.random.default_rng(seed)
+    n = len(prepared)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+    arrive -= arrive[0]
+    ids, done, roots, errors = [None] * n, [None] * n, {}, []
+    ready = [threading.Event() for _ in range(n)]
+
+    def releaser():
+        try:
+            for i in range(n):
+                ready[i].wait()
+                r = srv.wait(ids[i], timeout=600.0)
+                if r["status"] != 0:
+                    raise RuntimeError(f"job {i} status {r['status']}")
+                done[i] = r
+                if i in keep:
+                    roots[i] = srv.result(ids[i])
+                srv.release(ids[i])
+        except Exception as e:  # reported by the caller
+            errors.append(e)
+            for e2 in ready:
+                e2.set()
+
+    th = threading.Thread(target=releaser)
+    th.start()
+    t0 = srv.clock()
+    lag = []
+    for i, pj in enumerate(prepared):
+        while srv.clock() - t0 < arrive[i]:
+            time.sleep(min(2e-4, max(0.0, arrive[i] - (srv.clock() - t0))))
+        ids[i] = srv.submit_prepared(pj)
+        lag.append(srv.clock() - t0 - arrive[i])
+        ready[i].set()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    if errors:
+        raise errors[0]
+    lat = np.array([done[i]["done_s"] - (t0 + arrive[i]) for i in range(n)])
+    span = max(d["done_s"] for d in done) - t0
+    out = {"jobs": n, "offered_rate": rate if rate > 0 else "all at t=0", "jobs_per_s": round(n / span, 3),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "mean_submit_lag_ms": round(float(np.mean(lag)) * 1e3, 3),
+           "ticks": st["ticks"], "pool_peak_gib": round(st["pool_peak_bytes"] / 2 ** 30, 2)}
+    return out, roots
+
+
+def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, threads: int):
+    """Continuous-batching server vs the plain per-call loop on contraction-tree
+    jobs.  dist "spec": 20-300 contractions per job, every tensor's size
+    log-uniform in 2^8..2^28 elements (leaves up to 2^29); "small": the same
+    trees with sizes in 2^4..2^16."""
+    import numpy as np
+    import torch
+
+    from muscle.serve import LeafBank, Server, tree_nodes
+    from muscle.workloads import contraction_tree
+    size = (8, 28) if dist == "spec" else (4, 16)
+    specs = [contraction_tree(seed + i, size_log2=size) for i in range(n_jobs)]
+    bank = LeafBank(seed=seed)
+    bank.reserve(specs)
+    rng = np.random.default_rng(seed)
+    proto = Server(pool_gib=1.0 / 1024)
+    prepared = [proto.prepare(bank.leaves(s, rng), tree_nodes(s)) for s in specs]
+    proto.close()
+    torch.cuda.synchronize()
+    free, total = torch.cuda.mem_get_info()
+    pool = int(min(free - (8 << 30), 288e9 - bank.bytes()))
+    flops = [sum(8.0 * 2 ** len(set(s.leaves[a] if a < len(s.leaves) else s.nodes[a - len(s.leaves)][2]) |
+                              set(s.leaves[b] if b < len(s.leaves) else s.nodes[b - len(s.leaves)][2]))
+                 for a, b, _ in s.nodes) for s in specs]
+    res = {"distribution": f"trees of 20-300 binary_einsums, dims 2, tensor sizes log-uniform in 2^{size[0]}.."
+                           f"2^{size[1]}, ComplexF32/ComplexF64 per job",
+           "jobs": n_jobs, "c64_jobs": sum(s.dtype == "c64" for s in specs),
+           "nodes": sum(len(s.nodes) for s in specs), "tflop_total": round(sum(flops) / 1e12, 3),
+           "leaf_bank_gib": round(bank.bytes() / 2 ** 30, 2), "pool_gib": round(pool / 2 ** 30, 2)}
+    # sample of jobs checked against the oracle: the cheapest of each dtype
+    order = sorted(range(n_jobs), key=lambda i: flops[i])
+    keep = [next(i for i in order if specs[i].dtype == dt) for dt in ("c64", "c32")][:check_jobs]
+    log(f"served[{dist}]: {res}")
+    serial, _ = served_run(prepared, "loop", 0, pool, seed, ())
+    log(f"served[{dist}] plain loop: {serial}")
+    res["plain_loop"] = serial
+    res["batched"] = []
+    roots = {}
+    for rate in rates:
+        r, got = served_run(prepared, "batched", rate, pool, seed + 1, keep if not roots else ())
+        roots = roots or got
+        r["tflops"] = round(sum(flops) / (n_jobs / r["jobs_per_s"]) / 1e12, 3)
+        log(f"served[{dist}] batched rate {rate}: {r}")
+        res["batched"].append(r)
+    best = max(r["jobs_per_s"] for r in res["batched"])
+    res["speedup_vs_plain_loop"] = round(best / serial["jobs_per_s"], 3)
+    if keep and threads > 0:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import backend_base as bb
+        errs = {}
+        with threadpool_limits(threads):
+            for i in keep:
+                s = specs[i]
+                t = time.perf_counter()
+                vals = [x.parent().cpu().numpy().astype(np.complex128) for x in prepared[i].leaves]
+                labs = [list(l) for l in s.leaves]
+                for a, b, o in s.nodes:
+                    vals.append(bb.binary_einsum_general(list(o), vals[a], labs[a], vals[b], labs[b]))
+                    labs.append(list(o))
+                got = roots[i].parent().cpu().numpy()
+                errs[f"job{i}_{s.dtype}"] = {"rel_err_vs_cpu_oracle": rel(got, vals[-1]),
+                                             "nodes": len(s.nodes), "tflop": round(flops[i] / 1e12, 4),
+                                             "oracle_s": round(time.perf_counter() - t, 2)}
+                del vals
+        res["final_tensor_error"] = errs
+    return res
+
+
 def measured_traffic(precision: str):
     """HBM bytes per launch of the headline kernel, from the committed rocprofv3
     FETCH_SIZE + WRITE_SIZE summary (profiles/traffic_config2_c64.txt), or None."""
@@ -245,6 +372,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=10000)
     ap.add_argument("--batch-budget-gib", type=float, default=96.0)
+    ap.add_argument("--served-jobs", type=int, default=32, help="jobs per rate, full-size trees")
+    ap.add_argument("--served-small-jobs", type=int, default=200, help="jobs per rate, small trees")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="after the headline, copy this many GiB device-to-device 3 times (a kernel of known "
                          "traffic to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -368,12 +497,16 @@ def main() -> int:
             extra["batched_c64"] = {"error": repr(e)}
             log(f"batched failed: {e!r}")
     if want("served"):
-        try:
-            from muscle.serve import bench_served
-        except ImportError:
-            bench_served = None
-        if bench_served is not None:
-            extra["served"] = bench_served(seed=0, log=log)
+        for dist, n_jobs in (("small", args.served_small_jobs), ("spec", args.served_jobs)):
+            if n_jobs <= 0:
+                continue
+            try:
+                extra[f"served_{dist}"] = bench_served(dist, n_jobs, [50, 200, 1000, 2000], 1000, 2,
+                                                       0 if args.no_cpu_baseline else threads)
+            except Exception as e:  # reported in the JSON, not hidden
+                extra[f"served_{dist}"] = {"error": repr(e)}
+                log(f"served {dist} failed: {e!r}")
+            torch.cuda.empty_cache()
 
     peak = PEAK_TFLOPS["fp64"]
     out = {

@@ -242,6 +242,78 @@ MUSCLE_HIP_API int muscle_hip_tuning_load(muscle_hip_handle_t handle, const char
 /* Size class of a contraction of fused extents m x n (the key of the table). */
 MUSCLE_HIP_API int muscle_hip_tuning_class(int64_t m, int64_t n);
 
+/* --- serving: contraction-tree jobs with continuous batching ------------ */
+
+typedef struct muscle_hip_server_s* muscle_hip_server_t;
+
+/* One contraction-tree job.  Operands are numbered 0..n_leaves-1 for the
+ * leaves (device tensors owned by the caller, valid until the job is done)
+ * and n_leaves + i for the output of node i.  Node i contracts operands lhs[i]
+ * and rhs[i] (binary_einsum semantics) into a column-major tensor whose modes
+ * are node_modes[node_off[i] .. node_off[i+1]), with extents
+ * label_extents[mode].  Nodes are in topological order and the last node is
+ * the root.  Every tensor of a job has the leaves' element type.  These are
+ * the pairwise contractions a tensor-network caller issues as successive
+ * binary_einsum calls (src/Operations/binary_einsum.jl:33-51). */
+typedef struct {
+  int32_t n_leaves;
+  int32_t n_nodes;
+  const muscle_hip_tensor_t* leaves;
+  const int32_t* lhs;
+  const int32_t* rhs;
+  const int32_t* node_off;      /* [n_nodes + 1] */
+  const int32_t* node_modes;
+  int32_t n_labels;
+  const int64_t* label_extents; /* [n_labels] */
+} muscle_hip_job_t;
+
+typedef struct {
+  int32_t status;       /* muscle_hip_status_t of the job (NOT_READY while running) */
+  int32_t dtype;
+  void* result;         /* device address of the root tensor, column-major in the root's modes */
+  int64_t result_elems;
+  double submit_s;      /* server clock at submission, admission and completion */
+  double admit_s;
+  double done_s;
+  int64_t ticks;        /* launches the job's contractions took part in */
+} muscle_hip_job_result_t;
+
+typedef struct {
+  int64_t submitted, completed, rejected;
+  int64_t ticks;             /* batched launches (mode 0) or plain calls (mode 1) */
+  int64_t launched_pairs;
+  int64_t pool_bytes, pool_peak_bytes;
+  int64_t active_jobs, waiting_jobs;
+} muscle_hip_server_stats_t;
+
+/* Creates a server with its own stream, plan cache and a device pool of
+ * pool_bytes for intermediates and results.  mode 0: continuous batching --
+ * jobs are admitted in arrival order while their whole intermediate footprint
+ * fits in the pool next to the admitted jobs (intermediates are freed, and
+ * the reservation shrinks, as soon as their consumer completes), and every
+ * scheduler tick launches, as one persistent batched kernel per element
+ * type, all contractions of all admitted jobs whose operands have been
+ * launched.  mode 1: the plain per-call loop -- one job at a time, one
+ * binary_einsum launch per node.  max_active_jobs bounds admission. */
+MUSCLE_HIP_API int muscle_hip_server_create(int64_t pool_bytes, int max_active_jobs, int mode,
+                                            muscle_hip_server_t* server);
+/* Copies the job description and queues it; REJECTED if it can never fit. */
+MUSCLE_HIP_API int muscle_hip_server_submit(muscle_hip_server_t server, const muscle_hip_job_t* job,
+                                            int64_t* job_id);
+/* Waits up to timeout_s (< 0: forever) for a job; NOT_READY on timeout. */
+MUSCLE_HIP_API int muscle_hip_server_wait(muscle_hip_server_t server, int64_t job_id, double timeout_s,
+                                          muscle_hip_job_result_t* result);
+/* Copies a finished job's root tensor (result_elems elements) to dst (device
+ * or host memory), synchronously. */
+MUSCLE_HIP_API int muscle_hip_server_copy_result(muscle_hip_server_t server, int64_t job_id, void* dst);
+/* Returns a finished job's result memory to the pool. */
+MUSCLE_HIP_API int muscle_hip_server_release(muscle_hip_server_t server, int64_t job_id);
+MUSCLE_HIP_API int muscle_hip_server_get_stats(muscle_hip_server_t server, muscle_hip_server_stats_t* stats);
+/* Seconds on the server's clock (the clock of the job timestamps). */
+MUSCLE_HIP_API double muscle_hip_server_clock(muscle_hip_server_t server);
+/* Drains running jobs and frees the server. */
+MUSCLE_HIP_API int muscle_hip_server_destroy(muscle_hip_server_t server);
+
 /* Host-only: lower a binary contraction and report the fused GEMM shape and
  * the kernel the library would pick, without touching the GPU.  Replaces no
  * reference function; it exposes the index classification of

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "handle.hpp"
+#include "lowering.hpp"
 #include "kernels.hpp"
 
 namespace mh {
@@ -68,6 +69,34 @@ struct Batch {
   ~Batch();
 };
 
+// Location of one two-level table inside a batch's table arena.
+struct TabRef {
+  size_t lo, hi;
+  uint32_t P, single;
+};
+
+// One pair after host planning.
+struct PlannedPair {
+  ContractionDesc d;
+  TabRef t[9];
+  int tile;
+  int64_t cost;
+  const void* a;
+  const void* b;
+  void* c;
+};
+
+// Device layout of a planned batch: pairs | items | counter | tables.
+struct BatchLayout {
+  size_t pairs, items, counter, tab, total;
+};
+
+void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
+                const muscle_hip_tensor_t* bs, const int32_t* tiles, bool keep_descs, Batch& B,
+                std::vector<PlannedPair>& lw);
+BatchLayout batch_layout(const Batch& B, int64_t n);
+void place_batch(Batch& B, const std::vector<PlannedPair>& lw, char* base, char* staging, hipStream_t s);
+void batch_launch(Batch& b, hipStream_t s);
 std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
                                    const muscle_hip_tensor_t* bs, const int32_t* tiles);
 void batch_run(Batch& b, int mode);

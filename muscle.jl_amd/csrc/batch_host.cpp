@@ -15,7 +15,8 @@
 namespace mh {
 
 Batch::~Batch() {
-  if (h) (void)hipStreamSynchronize(h->stream);
+  // batches with their own device memory may still be read by queued work
+  if (h && (dmem || exec)) (void)hipStreamSynchronize(h->stream);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -36,11 +37,6 @@ int default_tile(int64_t m, int64_t n) {
 }
 
 size_t al(size_t v) { return (v + 255) & ~size_t(255); }
-
-struct TabRef {
-  size_t lo, hi;
-  uint32_t P, single;
-};
 
 TabRef add_table(std::vector<int64_t>& h, const std::vector<Dim>& dims, int slot) {
   std::vector<Dim> lo, hi;
@@ -80,35 +76,30 @@ void copy_desc(Batch& b, const muscle_hip_tensor_t& t, std::vector<muscle_hip_te
 
 int batch_tuning_class(int64_t m, int64_t n) { return tuning_class(m, n); }
 
-std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
-                                   const muscle_hip_tensor_t* bs, const int32_t* tiles) {
-  std::unique_ptr<Batch> B(new Batch());
-  B->h = &h;
+// Host phase: lowering, compact tables, tile choice, work list.
+void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
+                const muscle_hip_tensor_t* bs, const int32_t* tiles, bool keep_descs, Batch& B,
+                std::vector<PlannedPair>& lw) {
+  B.h = &h;
   if (n < 0) fail(MUSCLE_HIP_INVALID_ARGUMENT, "negative batch size");
   if (n > 0 && (!cs || !as || !bs)) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null tensor arrays");
-  struct Lowered {
-    ContractionDesc d;
-    TabRef t[9];
-    int tile;
-    int64_t cost;
-  };
-  std::vector<Lowered> lw((size_t)n);
+  lw.assign((size_t)n, PlannedPair());
   const int T3[9][2] = {{0, 0}, {2, 0}, {2, 1}, {1, 1}, {0, 2}, {1, 2}, {3, 0}, {3, 1}, {3, 2}};
   for (int64_t i = 0; i < n; ++i) {
     check_tensor(&cs[i], "C");
     check_tensor(&as[i], "A");
     check_tensor(&bs[i], "B");
-    if (i == 0) B->dtype = cs[0].dtype;
-    if (cs[i].dtype != B->dtype || as[i].dtype != B->dtype || bs[i].dtype != B->dtype)
+    if (i == 0) B.dtype = cs[0].dtype;
+    if (cs[i].dtype != B.dtype || as[i].dtype != B.dtype || bs[i].dtype != B.dtype)
       fail(MUSCLE_HIP_UNSUPPORTED, "batched contractions need one element type for every operand");
-    Lowered& L = lw[(size_t)i];
+    PlannedPair& L = lw[(size_t)i];
     L.d = lower_binary(cs[i], as[i], bs[i]);
     const ContractionDesc& d = L.d;
     const int64_t lim = int64_t(1) << 31;
     if (d.M >= lim || d.N >= lim || d.K >= lim || d.B >= lim)
       fail(MUSCLE_HIP_UNSUPPORTED, "batched pair with an index class of 2^31 or more elements");
     const std::vector<Dim>* cls[4] = {&d.m, &d.n, &d.k, &d.batch};
-    for (int t = 0; t < 9; ++t) L.t[t] = add_table(B->htab, *cls[T3[t][0]], T3[t][1]);
+    for (int t = 0; t < 9; ++t) L.t[t] = add_table(B.htab, *cls[T3[t][0]], T3[t][1]);
     int tile = tiles ? tiles[i] : MUSCLE_HIP_TILE_AUTO;
     if (tile < 0 || tile >= MUSCLE_HIP_TILE_COUNT) {
       const int tc = h.tune[tuning_class(d.M, d.N)];
@@ -116,24 +107,29 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
     }
     L.tile = tile;
     L.cost = (d.K + 15) / 16;
-    copy_desc(*B, cs[i], B->cs);
-    copy_desc(*B, as[i], B->as);
-    copy_desc(*B, bs[i], B->bs);
-    const bool cp = is_complex(B->dtype);
-    B->flops += (cp ? 8.0 : 2.0) * (double)d.M * (double)d.N * (double)d.K * (double)d.B;
+    L.a = d.swapped ? bs[i].data : as[i].data;
+    L.b = d.swapped ? as[i].data : bs[i].data;
+    L.c = cs[i].data;
+    if (keep_descs) {
+      copy_desc(B, cs[i], B.cs);
+      copy_desc(B, as[i], B.as);
+      copy_desc(B, bs[i], B.bs);
+    }
+    const bool cp = is_complex(B.dtype);
+    B.flops += (cp ? 8.0 : 2.0) * (double)d.M * (double)d.N * (double)d.K * (double)d.B;
     for (const muscle_hip_tensor_t* t : {&cs[i], &as[i], &bs[i]}) {
       double e = 1;
       for (int r = 0; r < t->rank; ++r) e *= (double)t->extents[r];
-      B->bytes += e * (double)dtype_size(t->dtype);
+      B.bytes += e * (double)dtype_size(t->dtype);
     }
   }
   // point the stored descriptors at the owned copies
-  for (size_t i = 0; i < B->cs.size(); ++i) {
-    muscle_hip_tensor_t* ts[3] = {&B->cs[i], &B->as[i], &B->bs[i]};
+  for (size_t i = 0; i < B.cs.size(); ++i) {
+    muscle_hip_tensor_t* ts[3] = {&B.cs[i], &B.as[i], &B.bs[i]};
     for (int j = 0; j < 3; ++j) {
-      ts[j]->modes = B->modes[3 * i + j].data();
-      ts[j]->extents = B->ext[3 * i + j].data();
-      ts[j]->strides = B->str[3 * i + j].data();
+      ts[j]->modes = B.modes[3 * i + j].data();
+      ts[j]->extents = B.ext[3 * i + j].data();
+      ts[j]->strides = B.str[3 * i + j].data();
     }
   }
   // work list: pairs with the longest tiles (largest K) first
@@ -143,39 +139,53 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
                    [&](int64_t x, int64_t y) { return lw[(size_t)x].cost > lw[(size_t)y].cost; });
   bool used[MUSCLE_HIP_TILE_COUNT] = {false, false, false};
   for (int64_t i : order) {
-    const Lowered& L = lw[(size_t)i];
+    const PlannedPair& L = lw[(size_t)i];
     if (L.d.empty) continue;
     const TileShape& ts = kTileShapes[L.tile];
     const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
     const int64_t items = tm * tn * L.d.B;
     if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
-    for (int64_t t = 0; t < items; ++t) B->hitems.push_back(make_int2((int)i, (int)t));
+    for (int64_t t = 0; t < items; ++t) B.hitems.push_back(make_int2((int)i, (int)t));
     used[L.tile] = true;
   }
-  // device arena: pairs | items | counter | tables
-  const size_t o_pairs = 0, o_items = al(sizeof(BatchPair) * (size_t)std::max<int64_t>(n, 1));
-  const size_t o_counter = o_items + al(sizeof(int2) * std::max<size_t>(B->hitems.size(), 1));
-  const size_t o_tab = o_counter + 256;
-  const size_t total = o_tab + al(B->htab.size() * sizeof(int64_t));
-  if (hipMalloc(&B->dmem, total) != hipSuccess) {
-    (void)hipGetLastError();
-    B->dmem = nullptr;
-    fail(MUSCLE_HIP_OUT_OF_MEMORY, "batch descriptor allocation failed");
+  int cus = h.cus;
+  if (cus <= 0) {
+    MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
+    h.cus = cus;
   }
-  char* base = (char*)B->dmem;
-  B->dpairs = (BatchPair*)(base + o_pairs);
-  B->ditems = (int2*)(base + o_items);
-  B->dcounter = (int32_t*)(base + o_counter);
-  const int64_t* dtab = (const int64_t*)(base + o_tab);
-  B->hpairs.resize((size_t)n);
+  B.workgroups = std::max(1, cus) * kBatchWgPerCu;
+  const bool dbl = is_double(B.dtype), cp = is_complex(B.dtype);
+  for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
+    if (used[t]) B.lds = std::max(B.lds, batch_lds_bytes(dbl, cp, t));
+}
+
+BatchLayout batch_layout(const Batch& B, int64_t n) {
+  BatchLayout l;
+  l.pairs = 0;
+  l.items = al(sizeof(BatchPair) * (size_t)std::max<int64_t>(n, 1));
+  l.counter = l.items + al(sizeof(int2) * std::max<size_t>(B.hitems.size(), 1));
+  l.tab = l.counter + 256;
+  l.total = l.tab + al(std::max<size_t>(B.htab.size(), 1) * sizeof(int64_t));
+  return l;
+}
+
+// Device phase: descriptors with device table pointers, copied to `base`
+// (synchronously, or through `staging` pinned memory on stream s).
+void place_batch(Batch& B, const std::vector<PlannedPair>& lw, char* base, char* staging, hipStream_t s) {
+  const int64_t n = (int64_t)lw.size();
+  const BatchLayout l = batch_layout(B, n);
+  B.dpairs = (BatchPair*)(base + l.pairs);
+  B.ditems = (int2*)(base + l.items);
+  B.dcounter = (int32_t*)(base + l.counter);
+  const int64_t* dtab = (const int64_t*)(base + l.tab);
+  B.hpairs.resize((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
-    const Lowered& L = lw[(size_t)i];
-    BatchPair& P = B->hpairs[(size_t)i];
+    const PlannedPair& L = lw[(size_t)i];
+    BatchPair& P = B.hpairs[(size_t)i];
     std::memset(&P, 0, sizeof P);
-    const bool sw = L.d.swapped;
-    P.A = sw ? bs[i].data : as[i].data;
-    P.B = sw ? as[i].data : bs[i].data;
-    P.C = cs[i].data;
+    P.A = L.a;
+    P.B = L.b;
+    P.C = L.c;
     P.M = (int32_t)L.d.M;
     P.N = (int32_t)L.d.N;
     P.K = (int32_t)L.d.K;
@@ -193,22 +203,36 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
     P.a_kmajor = minst(L.d.k, 0) < minst(L.d.m, 0);
     P.b_kmajor = minst(L.d.k, 1) < minst(L.d.n, 1);
     OffTab* tabs[9] = {&P.rowA, &P.colB, &P.kA, &P.kB, &P.rowC, &P.colC, &P.bA, &P.bB, &P.bC};
-    // tables were appended in T3 order: rowA, kA, kB, colB, rowC, colC, bA, bB, bC
+    // tables were appended in the order rowA, kA, kB, colB, rowC, colC, bA, bB, bC
     const int map[9] = {0, 3, 1, 2, 4, 5, 6, 7, 8};
     for (int t = 0; t < 9; ++t) *tabs[t] = make_tab(dtab, L.t[map[t]]);
   }
-  MH_HIP_CHECK(hipMemcpy(base + o_pairs, B->hpairs.data(), sizeof(BatchPair) * (size_t)n, hipMemcpyHostToDevice));
-  if (!B->hitems.empty())
-    MH_HIP_CHECK(hipMemcpy(base + o_items, B->hitems.data(), sizeof(int2) * B->hitems.size(),
-                           hipMemcpyHostToDevice));
-  if (!B->htab.empty())
-    MH_HIP_CHECK(hipMemcpy(base + o_tab, B->htab.data(), B->htab.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  int cus = 0;
-  MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
-  B->workgroups = std::max(1, cus) * kBatchWgPerCu;
-  const bool dbl = is_double(B->dtype), cp = is_complex(B->dtype);
-  for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
-    if (used[t]) B->lds = std::max(B->lds, batch_lds_bytes(dbl, cp, t));
+  const size_t np = sizeof(BatchPair) * (size_t)n, ni = sizeof(int2) * B.hitems.size(),
+               nt = B.htab.size() * sizeof(int64_t);
+  if (staging) {
+    std::memcpy(staging + l.pairs, B.hpairs.data(), np);
+    if (ni) std::memcpy(staging + l.items, B.hitems.data(), ni);
+    if (nt) std::memcpy(staging + l.tab, B.htab.data(), nt);
+    MH_HIP_CHECK(hipMemcpyAsync(base, staging, l.total, hipMemcpyHostToDevice, s));
+  } else {
+    if (np) MH_HIP_CHECK(hipMemcpy(base + l.pairs, B.hpairs.data(), np, hipMemcpyHostToDevice));
+    if (ni) MH_HIP_CHECK(hipMemcpy(base + l.items, B.hitems.data(), ni, hipMemcpyHostToDevice));
+    if (nt) MH_HIP_CHECK(hipMemcpy(base + l.tab, B.htab.data(), nt, hipMemcpyHostToDevice));
+  }
+}
+
+std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
+                                   const muscle_hip_tensor_t* bs, const int32_t* tiles) {
+  std::unique_ptr<Batch> B(new Batch());
+  std::vector<PlannedPair> lw;
+  plan_batch(h, n, cs, as, bs, tiles, true, *B, lw);
+  const BatchLayout l = batch_layout(*B, n);
+  if (hipMalloc(&B->dmem, l.total) != hipSuccess) {
+    (void)hipGetLastError();
+    B->dmem = nullptr;
+    fail(MUSCLE_HIP_OUT_OF_MEMORY, "batch descriptor allocation failed");
+  }
+  place_batch(*B, lw, (char*)B->dmem, nullptr, nullptr);
   return B;
 }
 

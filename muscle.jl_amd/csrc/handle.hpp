@@ -76,6 +76,7 @@ struct Plan {
 struct Handle {
   static constexpr size_t kMaxPlans = 8192;
   int device = 0;
+  int cus = 0;  // compute units of the device (queried once)
   hipStream_t stream = nullptr;
   std::mutex mu;
   std::unordered_map<std::string, std::unique_ptr<Plan>> cache;

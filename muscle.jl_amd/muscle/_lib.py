@@ -57,6 +57,44 @@ class BatchInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_leaves", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int32),
+        ("leaves", ctypes.POINTER(TensorDesc)),
+        ("lhs", ctypes.POINTER(ctypes.c_int32)),
+        ("rhs", ctypes.POINTER(ctypes.c_int32)),
+        ("node_off", ctypes.POINTER(ctypes.c_int32)),
+        ("node_modes", ctypes.POINTER(ctypes.c_int32)),
+        ("n_labels", ctypes.c_int32),
+        ("label_extents", ctypes.POINTER(ctypes.c_int64)),
+    ]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("result", ctypes.c_void_p),
+        ("result_elems", ctypes.c_int64),
+        ("submit_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("ticks", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "rejected", "ticks", "launched_pairs",
+                                              "pool_bytes", "pool_peak_bytes", "active_jobs", "waiting_jobs")]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Info(ctypes.Structure):
     _fields_ = [
         ("kernel", ctypes.c_int32),
@@ -108,6 +146,17 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_tuning_save": ([H, ctypes.c_char_p], ctypes.c_int),
         "muscle_hip_tuning_load": ([H, ctypes.c_char_p], ctypes.c_int),
         "muscle_hip_tuning_class": ([ctypes.c_int64, ctypes.c_int64], ctypes.c_int),
+        "muscle_hip_server_create": ([ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+                                     ctypes.c_int),
+        "muscle_hip_server_submit": ([ctypes.c_void_p, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int64)],
+                                     ctypes.c_int),
+        "muscle_hip_server_wait": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(JobResult)],
+                                   ctypes.c_int),
+        "muscle_hip_server_copy_result": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+        "muscle_hip_server_release": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+        "muscle_hip_server_get_stats": ([ctypes.c_void_p, ctypes.POINTER(ServerStats)], ctypes.c_int),
+        "muscle_hip_server_clock": ([ctypes.c_void_p], ctypes.c_double),
+        "muscle_hip_server_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "muscle_hip_describe": ([T, T, T, ctypes.c_int, ctypes.c_double, I, ctypes.POINTER(ctypes.c_int32),
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     }

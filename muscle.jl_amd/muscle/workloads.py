@@ -137,3 +137,106 @@ def pair_flops(pair, complex_=True) -> float:
     ia, ib, ic = pair
     labels = set(ia) | set(ib)
     return (8.0 if complex_ else 2.0) * 2.0 ** len(labels)
+
+
+# ------------------------------------------------------------ served contraction trees
+class TreeSpec:
+    """Index structure of one contraction-tree job.
+
+    leaves: list of index-label lists; nodes: list of (lhs, rhs, out_labels)
+    in topological order, operands numbered leaves first then nodes; the last
+    node is the root.  Every label has dimension 2; dtype is "c32" or "c64"."""
+
+    def __init__(self, leaves, nodes, dtype):
+        self.leaves, self.nodes, self.dtype = leaves, nodes, dtype
+
+    def sizes_log2(self):
+        return [len(n[2]) for n in self.nodes]
+
+
+def contraction_tree(seed: int, n_nodes=None, size_log2=(8, 28), max_contracted=10, hyper=(0, 1),
+                     dtype=None) -> TreeSpec:
+    """A seeded random contraction tree of circuit-like tensors (every index of
+    dimension 2) with 20-300 pairwise contractions and a ComplexF32 or
+    ComplexF64 dtype.  Built top-down from the root: every tensor of the tree
+    (intermediates and leaves) gets a rank drawn uniformly from size_log2, so
+    sizes are log-uniform in 2^8..2^28; an expansion splits a node's open
+    indices between its two children, keeps 0-1 of them as a hyper (batch)
+    index on both, and adds as many new contracted indices (1..max_contracted)
+    as the children's ranks need."""
+    rng = np.random.default_rng(seed)
+    lo, hi = size_log2
+    if n_nodes is None:
+        n_nodes = int(rng.integers(20, 301))
+    if dtype is None:
+        dtype = "c32" if rng.integers(0, 2) == 0 else "c64"
+    counter = [0]
+
+    def fresh(k):
+        out = list(range(counter[0], counter[0] + k))
+        counter[0] += k
+        return out
+
+    root = {"out": fresh(int(rng.integers(lo, hi + 1))), "kids": None}
+    frontier, internal = [root], []
+    while len(internal) < n_nodes:
+        node = frontier.pop(int(rng.integers(len(frontier))))
+        O = node["out"]
+        r = len(O)
+        h = min(int(rng.integers(hyper[0], hyper[1] + 1)), r)
+        H = [O[i] for i in rng.choice(r, h, replace=False)] if h else []
+        rest = [x for x in O if x not in H]
+        tl, tr = (int(x) for x in rng.integers(lo, hi + 1, 2))
+        c = int(np.clip(round((tl + tr - r - h) / 2), 1, max_contracted))
+        nl = int(np.clip(tl - c - h, 0, len(rest)))
+        C = fresh(c)
+        perm = [rest[i] for i in rng.permutation(len(rest))]
+        L, R = perm[:nl], perm[nl:]
+        left = {"out": [int(x) for x in rng.permutation(L + C + H)], "kids": None}
+        right = {"out": [int(x) for x in rng.permutation(R + C + H)], "kids": None}
+        node["kids"] = (left, right)
+        internal.append(node)
+        frontier += [left, right]
+    leaves = frontier
+    ids = {id(n): i for i, n in enumerate(leaves)}
+    nodes = []
+
+    def emit(n):
+        if n["kids"] is None:
+            return ids[id(n)]
+        a = emit(n["kids"][0])
+        b = emit(n["kids"][1])
+        nodes.append((a, b, list(n["out"])))
+        return len(leaves) + len(nodes) - 1
+
+    import sys
+    old = sys.getrecursionlimit()
+    sys.setrecursionlimit(max(old, 10000))
+    try:
+        emit(root)
+    finally:
+        sys.setrecursionlimit(old)
+    return TreeSpec([list(n["out"]) for n in leaves], nodes, dtype)
+
+
+def leaf_scale_exponents(spec: TreeSpec) -> list:
+    """Integer e per leaf such that leaves with iid entries of E|z|^2 = 4^-e keep
+    every node's expected E|z|^2 near 1.
+
+    A node contracting c indices of dimension 2 multiplies its operands'
+    second moments and 2^c; the factor 2^-c is shared equally by the leaves of
+    each child subtree (2^(-c/2) per subtree), and the per-leaf log2 sums are
+    rounded to the nearest even number (so the scale 2^-e is a power of two)."""
+    nl, nn = len(spec.leaves), len(spec.nodes)
+    labels = [set(l) for l in spec.leaves] + [set(n[2]) for n in spec.nodes]
+    count = [1] * nl + [0] * nn
+    for n, (a, b, _) in enumerate(spec.nodes):
+        count[nl + n] = count[a] + count[b]
+    share = [0.0] * (nl + nn)  # log2 second moment to add to every leaf below
+    for n in range(nn - 1, -1, -1):
+        a, b, _ = spec.nodes[n]
+        k = nl + n
+        c = len((labels[a] & labels[b]) - labels[k])
+        for ch in (a, b):
+            share[ch] += share[k] - c / 2.0 / count[ch]
+    return [int(round(-share[i] / 2.0)) for i in range(nl)]
