@@ -297,7 +297,11 @@ typedef struct {
  * binary_einsum launch per node.  max_active_jobs bounds admission. */
 MUSCLE_HIP_API int muscle_hip_server_create(int64_t pool_bytes, int max_active_jobs, int mode,
                                             muscle_hip_server_t* server);
-/* Copies the job description and queues it; REJECTED if it can never fit. */
+/* Copies the job description and queues it; REJECTED if its intermediates
+ * can never fit in the pool.  Its contractions are lowered and planned on the
+ * server's planner threads: one the library cannot run (or a job whose
+ * tables push it past the pool) ends the job with that status, which wait
+ * returns together with the message (muscle_hip_last_error). */
 MUSCLE_HIP_API int muscle_hip_server_submit(muscle_hip_server_t server, const muscle_hip_job_t* job,
                                             int64_t* job_id);
 /* Waits up to timeout_s (< 0: forever) for a job; NOT_READY on timeout. */
@@ -311,7 +315,8 @@ MUSCLE_HIP_API int muscle_hip_server_release(muscle_hip_server_t server, int64_t
 MUSCLE_HIP_API int muscle_hip_server_get_stats(muscle_hip_server_t server, muscle_hip_server_stats_t* stats);
 /* Seconds on the server's clock (the clock of the job timestamps). */
 MUSCLE_HIP_API double muscle_hip_server_clock(muscle_hip_server_t server);
-/* Drains running jobs and frees the server. */
+/* Drains the admitted jobs and frees the server; jobs still waiting for
+ * admission end with MUSCLE_HIP_HIP_ERROR. */
 MUSCLE_HIP_API int muscle_hip_server_destroy(muscle_hip_server_t server);
 
 /* Host-only: lower a binary contraction and report the fused GEMM shape and

@@ -79,11 +79,13 @@ struct TabRef {
 struct PlannedPair {
   ContractionDesc d;
   TabRef t[9];
-  int tile;
-  int64_t cost;
-  const void* a;
-  const void* b;
-  void* c;
+  int tile = 0;
+  int a_kmajor = 0, b_kmajor = 0;
+  int64_t cost = 0;
+  const void* a = nullptr;  // kernel operands (after the lowering's A/B exchange)
+  const void* b = nullptr;
+  void* c = nullptr;
+  const int64_t* dtab = nullptr;  // device tables of this pair; null: the batch's own table region
 };
 
 // Device layout of a planned batch: pairs | items | counter | tables.
@@ -94,8 +96,20 @@ struct BatchLayout {
 void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
                 const muscle_hip_tensor_t* bs, const int32_t* tiles, bool keep_descs, Batch& B,
                 std::vector<PlannedPair>& lw);
+// Tables and tile of one pair whose lowering is in L.d: the nine offset
+// tables are appended to htab (L.t holds their positions); tile < 0 picks the
+// tuned or default shape for the pair's size class.
+// single_max: classes up to this many points get one-level tables.
+void plan_pair(const Handle& h, PlannedPair& L, std::vector<int64_t>& htab, int tile,
+               int64_t single_max = kSingleLevelMax);
+// Batches planned per scheduler tick use two-level tables from this many
+// points on, which keeps the tables (host time and upload) small at one more
+// lookup per index.
+constexpr int64_t kTickSingleMax = 256;
+// Work list (largest K first), workgroup count and LDS size of the pairs in lw.
+void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw);
 BatchLayout batch_layout(const Batch& B, int64_t n);
-void place_batch(Batch& B, const std::vector<PlannedPair>& lw, char* base, char* staging, hipStream_t s);
+void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base, char* staging, hipStream_t s);
 void batch_launch(Batch& b, hipStream_t s);
 std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
                                    const muscle_hip_tensor_t* bs, const int32_t* tiles);

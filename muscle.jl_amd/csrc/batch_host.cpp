@@ -38,10 +38,10 @@ int default_tile(int64_t m, int64_t n) {
 
 size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
-TabRef add_table(std::vector<int64_t>& h, const std::vector<Dim>& dims, int slot) {
+TabRef add_table(std::vector<int64_t>& h, const std::vector<Dim>& dims, int slot, int64_t single_max) {
   std::vector<Dim> lo, hi;
   int64_t P;
-  split_dims(dims, lo, hi, P);
+  split_dims(dims, lo, hi, P, single_max);
   TabRef r;
   const std::vector<int64_t> tl = offset_table(lo, slot), th = offset_table(hi, slot);
   r.lo = h.size();
@@ -76,6 +76,63 @@ void copy_desc(Batch& b, const muscle_hip_tensor_t& t, std::vector<muscle_hip_te
 
 int batch_tuning_class(int64_t m, int64_t n) { return tuning_class(m, n); }
 
+void plan_pair(const Handle& h, PlannedPair& L, std::vector<int64_t>& htab, int tile, int64_t single_max) {
+  const int T3[9][2] = {{0, 0}, {2, 0}, {2, 1}, {1, 1}, {0, 2}, {1, 2}, {3, 0}, {3, 1}, {3, 2}};
+  const ContractionDesc& d = L.d;
+  const int64_t lim = int64_t(1) << 31;
+  if (d.M >= lim || d.N >= lim || d.K >= lim || d.B >= lim)
+    fail(MUSCLE_HIP_UNSUPPORTED, "batched pair with an index class of 2^31 or more elements");
+  const std::vector<Dim>* cls[4] = {&d.m, &d.n, &d.k, &d.batch};
+  for (int t = 0; t < 9; ++t) L.t[t] = add_table(htab, *cls[T3[t][0]], T3[t][1], single_max);
+  if (tile < 0 || tile >= MUSCLE_HIP_TILE_COUNT) {
+    const int tc = h.tune[tuning_class(d.M, d.N)];
+    tile = tc >= 0 ? tc : default_tile(d.M, d.N);
+  }
+  L.tile = tile;
+  L.cost = (d.K + 15) / 16;
+  auto minst = [](const std::vector<Dim>& v, int w) {
+    int64_t b = INT64_MAX;
+    for (auto& x : v)
+      if (x.s[w] != 0) b = std::min(b, x.s[w] < 0 ? -x.s[w] : x.s[w]);
+    return b;
+  };
+  L.a_kmajor = minst(d.k, 0) < minst(d.m, 0);
+  L.b_kmajor = minst(d.k, 1) < minst(d.n, 1);
+}
+
+void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw) {
+  const int64_t n = (int64_t)lw.size();
+  B.hitems.clear();
+  B.lds = 0;
+  // work list: pairs with the longest tiles (largest K) first
+  std::vector<int64_t> order((size_t)n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t x, int64_t y) { return lw[(size_t)x]->cost > lw[(size_t)y]->cost; });
+  bool used[MUSCLE_HIP_TILE_COUNT] = {false, false, false};
+  const bool cp = is_complex(B.dtype);
+  for (int64_t i : order) {
+    const PlannedPair& L = *lw[(size_t)i];
+    if (L.d.empty) continue;
+    B.flops += (cp ? 8.0 : 2.0) * (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B;
+    const TileShape& ts = kTileShapes[L.tile];
+    const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
+    const int64_t items = tm * tn * L.d.B;
+    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
+    for (int64_t t = 0; t < items; ++t) B.hitems.push_back(make_int2((int)i, (int)t));
+    used[L.tile] = true;
+  }
+  int cus = h.cus;
+  if (cus <= 0) {
+    MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
+    h.cus = cus;
+  }
+  B.workgroups = std::max(1, cus) * kBatchWgPerCu;
+  const bool dbl = is_double(B.dtype);
+  for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
+    if (used[t]) B.lds = std::max(B.lds, batch_lds_bytes(dbl, cp, t));
+}
+
 // Host phase: lowering, compact tables, tile choice, work list.
 void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
                 const muscle_hip_tensor_t* bs, const int32_t* tiles, bool keep_descs, Batch& B,
@@ -84,7 +141,6 @@ void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscl
   if (n < 0) fail(MUSCLE_HIP_INVALID_ARGUMENT, "negative batch size");
   if (n > 0 && (!cs || !as || !bs)) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null tensor arrays");
   lw.assign((size_t)n, PlannedPair());
-  const int T3[9][2] = {{0, 0}, {2, 0}, {2, 1}, {1, 1}, {0, 2}, {1, 2}, {3, 0}, {3, 1}, {3, 2}};
   for (int64_t i = 0; i < n; ++i) {
     check_tensor(&cs[i], "C");
     check_tensor(&as[i], "A");
@@ -94,29 +150,15 @@ void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscl
       fail(MUSCLE_HIP_UNSUPPORTED, "batched contractions need one element type for every operand");
     PlannedPair& L = lw[(size_t)i];
     L.d = lower_binary(cs[i], as[i], bs[i]);
-    const ContractionDesc& d = L.d;
-    const int64_t lim = int64_t(1) << 31;
-    if (d.M >= lim || d.N >= lim || d.K >= lim || d.B >= lim)
-      fail(MUSCLE_HIP_UNSUPPORTED, "batched pair with an index class of 2^31 or more elements");
-    const std::vector<Dim>* cls[4] = {&d.m, &d.n, &d.k, &d.batch};
-    for (int t = 0; t < 9; ++t) L.t[t] = add_table(B.htab, *cls[T3[t][0]], T3[t][1]);
-    int tile = tiles ? tiles[i] : MUSCLE_HIP_TILE_AUTO;
-    if (tile < 0 || tile >= MUSCLE_HIP_TILE_COUNT) {
-      const int tc = h.tune[tuning_class(d.M, d.N)];
-      tile = tc >= 0 ? tc : default_tile(d.M, d.N);
-    }
-    L.tile = tile;
-    L.cost = (d.K + 15) / 16;
-    L.a = d.swapped ? bs[i].data : as[i].data;
-    L.b = d.swapped ? as[i].data : bs[i].data;
+    plan_pair(h, L, B.htab, tiles ? tiles[i] : MUSCLE_HIP_TILE_AUTO);
+    L.a = L.d.swapped ? bs[i].data : as[i].data;
+    L.b = L.d.swapped ? as[i].data : bs[i].data;
     L.c = cs[i].data;
     if (keep_descs) {
       copy_desc(B, cs[i], B.cs);
       copy_desc(B, as[i], B.as);
       copy_desc(B, bs[i], B.bs);
     }
-    const bool cp = is_complex(B.dtype);
-    B.flops += (cp ? 8.0 : 2.0) * (double)d.M * (double)d.N * (double)d.K * (double)d.B;
     for (const muscle_hip_tensor_t* t : {&cs[i], &as[i], &bs[i]}) {
       double e = 1;
       for (int r = 0; r < t->rank; ++r) e *= (double)t->extents[r];
@@ -132,31 +174,9 @@ void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscl
       ts[j]->strides = B.str[3 * i + j].data();
     }
   }
-  // work list: pairs with the longest tiles (largest K) first
-  std::vector<int64_t> order((size_t)n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int64_t x, int64_t y) { return lw[(size_t)x].cost > lw[(size_t)y].cost; });
-  bool used[MUSCLE_HIP_TILE_COUNT] = {false, false, false};
-  for (int64_t i : order) {
-    const PlannedPair& L = lw[(size_t)i];
-    if (L.d.empty) continue;
-    const TileShape& ts = kTileShapes[L.tile];
-    const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
-    const int64_t items = tm * tn * L.d.B;
-    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
-    for (int64_t t = 0; t < items; ++t) B.hitems.push_back(make_int2((int)i, (int)t));
-    used[L.tile] = true;
-  }
-  int cus = h.cus;
-  if (cus <= 0) {
-    MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
-    h.cus = cus;
-  }
-  B.workgroups = std::max(1, cus) * kBatchWgPerCu;
-  const bool dbl = is_double(B.dtype), cp = is_complex(B.dtype);
-  for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
-    if (used[t]) B.lds = std::max(B.lds, batch_lds_bytes(dbl, cp, t));
+  std::vector<const PlannedPair*> ptrs((size_t)n);
+  for (int64_t i = 0; i < n; ++i) ptrs[(size_t)i] = &lw[(size_t)i];
+  finish_batch(h, B, ptrs);
 }
 
 BatchLayout batch_layout(const Batch& B, int64_t n) {
@@ -171,7 +191,7 @@ BatchLayout batch_layout(const Batch& B, int64_t n) {
 
 // Device phase: descriptors with device table pointers, copied to `base`
 // (synchronously, or through `staging` pinned memory on stream s).
-void place_batch(Batch& B, const std::vector<PlannedPair>& lw, char* base, char* staging, hipStream_t s) {
+void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base, char* staging, hipStream_t s) {
   const int64_t n = (int64_t)lw.size();
   const BatchLayout l = batch_layout(B, n);
   B.dpairs = (BatchPair*)(base + l.pairs);
@@ -180,7 +200,7 @@ void place_batch(Batch& B, const std::vector<PlannedPair>& lw, char* base, char*
   const int64_t* dtab = (const int64_t*)(base + l.tab);
   B.hpairs.resize((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
-    const PlannedPair& L = lw[(size_t)i];
+    const PlannedPair& L = *lw[(size_t)i];
     BatchPair& P = B.hpairs[(size_t)i];
     std::memset(&P, 0, sizeof P);
     P.A = L.a;
@@ -194,18 +214,13 @@ void place_batch(Batch& B, const std::vector<PlannedPair>& lw, char* base, char*
     P.tiles_m = (int32_t)((L.d.M + ts.bm - 1) / ts.bm);
     P.tiles_n = (int32_t)((L.d.N + ts.bn - 1) / ts.bn);
     P.tile = L.tile;
-    auto minst = [](const std::vector<Dim>& v, int w) {
-      int64_t b = INT64_MAX;
-      for (auto& x : v)
-        if (x.s[w] != 0) b = std::min(b, x.s[w] < 0 ? -x.s[w] : x.s[w]);
-      return b;
-    };
-    P.a_kmajor = minst(L.d.k, 0) < minst(L.d.m, 0);
-    P.b_kmajor = minst(L.d.k, 1) < minst(L.d.n, 1);
+    P.a_kmajor = L.a_kmajor;
+    P.b_kmajor = L.b_kmajor;
     OffTab* tabs[9] = {&P.rowA, &P.colB, &P.kA, &P.kB, &P.rowC, &P.colC, &P.bA, &P.bB, &P.bC};
     // tables were appended in the order rowA, kA, kB, colB, rowC, colC, bA, bB, bC
     const int map[9] = {0, 3, 1, 2, 4, 5, 6, 7, 8};
-    for (int t = 0; t < 9; ++t) *tabs[t] = make_tab(dtab, L.t[map[t]]);
+    const int64_t* tb = L.dtab ? L.dtab : dtab;
+    for (int t = 0; t < 9; ++t) *tabs[t] = make_tab(tb, L.t[map[t]]);
   }
   const size_t np = sizeof(BatchPair) * (size_t)n, ni = sizeof(int2) * B.hitems.size(),
                nt = B.htab.size() * sizeof(int64_t);
@@ -232,7 +247,9 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
     B->dmem = nullptr;
     fail(MUSCLE_HIP_OUT_OF_MEMORY, "batch descriptor allocation failed");
   }
-  place_batch(*B, lw, (char*)B->dmem, nullptr, nullptr);
+  std::vector<const PlannedPair*> ptrs((size_t)n);
+  for (int64_t i = 0; i < n; ++i) ptrs[(size_t)i] = &lw[(size_t)i];
+  place_batch(*B, ptrs, (char*)B->dmem, nullptr, nullptr);
   return B;
 }
 

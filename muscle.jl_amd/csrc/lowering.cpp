@@ -117,7 +117,8 @@ int64_t product(const std::vector<Dim>& d) {
 
 }  // namespace
 
-void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim>& hi, int64_t& P) {
+void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim>& hi, int64_t& P,
+                int64_t single_max) {
   lo.clear();
   hi.clear();
   int64_t total = product(d);
@@ -126,7 +127,7 @@ void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim
     hi = d;
     return;
   }
-  if (total <= kSingleLevelMax) {  // small class: one level, P = total
+  if (total <= single_max) {  // small class: one level, P = total
     lo = d;
     P = total;
     return;

@@ -58,7 +58,8 @@ ElementwiseDesc lower_hadamard(const muscle_hip_tensor_t& c, const muscle_hip_te
 // when that gets closer; offsets then factor as lo-offset + hi-offset.
 // Classes of at most kSingleLevelMax points are not split (lo = d, P = total).
 constexpr int64_t kSingleLevelMax = 65536;
-void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim>& hi, int64_t& P);
+void split_dims(const std::vector<Dim>& d, std::vector<Dim>& lo, std::vector<Dim>& hi, int64_t& P,
+                int64_t single_max = kSingleLevelMax);
 
 // Offsets of every point of a mixed-radix loop nest (fastest-first dims),
 // using stride slot `which` (0 A, 1 B, 2 C).  Length = product of extents

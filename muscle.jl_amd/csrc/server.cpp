@@ -1,15 +1,20 @@
 // Contraction-tree serving with continuous batching.
 //
-// A scheduler thread owns a private handle (stream, plan cache) and a device
-// pool.  Jobs (trees of pairwise contractions) queue in arrival order and are
-// admitted while their intermediate footprint fits next to the admitted jobs;
-// each tick launches every contraction of every admitted job whose operands
-// have been launched (stream order makes them complete first) as one
-// persistent batched kernel per element type.  Up to kSlots ticks are in
-// flight; when a tick's event fires, its operands that have no consumer left
-// go back to the pool and the owning job's reservation shrinks, so waiting
-// jobs are admitted while others are still on the GPU.  Mode 1 runs the same
-// jobs as the plain per-call loop (one job at a time, one launch per node).
+// Jobs (trees of pairwise contractions) are compiled on planner threads as
+// they arrive: every contraction is lowered once, and the batched ones get
+// their offset tables and tile shape (one table arena per job).  A scheduler
+// thread owns a private handle (stream, plan cache) and a device pool.
+// Compiled jobs are admitted in arrival order while their footprint (all
+// intermediates, the result and the table arena) fits next to the admitted
+// jobs; admission uploads the job's tables.  Each tick launches every
+// contraction of every admitted job whose operands have been launched
+// (stream order makes them complete first): contractions big enough to fill
+// the GPU alone take the single-contraction kernel, all others share one
+// persistent batched launch per element type.  Up to kSlots ticks are in
+// flight; when a tick's event fires, operands that have no consumer left go
+// back to the pool and the owning job's reservation shrinks, so waiting jobs
+// are admitted while others are still on the GPU.  Mode 1 runs the same jobs
+// as the plain per-call loop (one job at a time, one launch per node).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,22 +108,33 @@ struct Node {
   int lhs, rhs;
   Operand out;
   size_t bytes = 0;
-  int consumers = 0;
+  int consumers = 0;       // readers that have not finished (the root: its caller)
+  std::vector<int> users;  // nodes that read this node's output
+  int pending = 0;         // node operands not launched yet
+  bool solo = false;       // runs as a single-contraction launch
+  PlannedPair plan;        // batched nodes: lowering, tables, tile
   bool launched = false, done = false, freed = false;
 };
+
+enum CompileState { kQueued = 0, kCompiled = 1, kFailed = 2 };
 
 struct Job {
   int64_t id = 0;
   int dtype = 0;
   std::vector<Operand> leaves;
   std::vector<Node> nodes;
-  size_t reserve = 0;   // bytes of all intermediates and the result
+  size_t reserve = 0;   // bytes of all intermediates, the result and the tables
   size_t reserved = 0;  // what is still reserved in the pool
   int status = MUSCLE_HIP_NOT_READY;
+  std::string error;
+  int compile = kQueued;
+  std::vector<int64_t> htab;  // offset tables of the batched nodes
+  char* dtab = nullptr;
+  size_t dtab_bytes = 0;
+  std::vector<int> ready;  // nodes whose operands have been launched
   double t_submit = 0, t_admit = 0, t_done = 0;
   int64_t ticks = 0;
   int remaining = 0;
-  bool released = false;
   const Operand& operand(int k) const {
     const int nl = (int)leaves.size();
     return k < nl ? leaves[(size_t)k] : nodes[(size_t)(k - nl)].out;
@@ -135,7 +151,6 @@ struct Slot {
   char* pinned = nullptr;
   size_t size = 0;
   hipEvent_t ev = nullptr;
-  bool busy = false;
   std::vector<TickEntry> entries;
   std::vector<std::unique_ptr<Batch>> batches;
   std::vector<void*> extra_dev;  // oversized tick descriptors
@@ -144,20 +159,8 @@ struct Slot {
 constexpr int kSlots = 3;
 constexpr size_t kSlotBytes = size_t(64) << 20;
 // index points (M*N*K*batch) from which a contraction runs on its own
-constexpr int64_t kSoloPoints = int64_t(1) << 29;
-
-// Number of index points of a pairwise contraction: the product of the
-// extents of the distinct modes of both operands.
-int64_t pair_points(const Operand& a, const Operand& b) {
-  std::vector<std::pair<int32_t, int64_t>> m;
-  for (const Operand* o : {&a, &b})
-    for (size_t i = 0; i < o->modes.size(); ++i) m.push_back({o->modes[i], o->ext[i]});
-  std::sort(m.begin(), m.end());
-  double n = 1;
-  for (size_t i = 0; i < m.size(); ++i)
-    if (i == 0 || m[i].first != m[i - 1].first) n *= (double)m[i].second;
-  return n > 9e18 ? INT64_MAX : (int64_t)n;
-}
+constexpr double kSoloPoints = double(int64_t(1) << 29);
+constexpr int kMaxPlanners = 8;
 
 }  // namespace
 
@@ -166,8 +169,9 @@ struct Server {
   int mode = 0;
   int max_active = 64;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::deque<std::shared_ptr<Job>> waiting;
+  std::condition_variable cv_work, cv_done, cv_plan;
+  std::deque<std::shared_ptr<Job>> waiting;   // arrival order, compiled or not
+  std::deque<std::shared_ptr<Job>> to_compile;
   std::vector<std::shared_ptr<Job>> active;
   std::unordered_map<int64_t, std::shared_ptr<Job>> jobs;
   Pool pool;
@@ -175,6 +179,7 @@ struct Server {
   Slot slots[kSlots];
   int slot_head = 0, slot_count = 0;  // in-flight ring
   std::thread worker;
+  std::vector<std::thread> planners;
   bool stop = false;
   int64_t next_id = 1;
   muscle_hip_server_stats_t st{};
@@ -186,10 +191,13 @@ struct Server {
   }
 
   void loop();
-  bool admit_locked();
+  void plan_loop();
+  void compile(Job& j);
+  std::vector<std::shared_ptr<Job>> admit_locked();
   bool launch_tick();
   bool complete_ticks(bool block);
   void finish_node(Job& j, int n);
+  void finish_job(Job& j, int status);
   void run_serial(Job& j);
 };
 
@@ -212,123 +220,206 @@ int64_t elems(const Operand& o) {
 
 }  // namespace
 
+// Lowers every contraction of a job; the batched ones get tables and a tile.
+void Server::compile(Job& j) {
+  for (Node& nd : j.nodes) {
+    muscle_hip_tensor_t c = nd.out.t;
+    c.data = (void*)256;  // addresses are bound at launch; the lowering reads shapes and strides only
+    nd.plan.d = lower_binary(c, j.operand(nd.lhs).t, j.operand(nd.rhs).t);
+    const ContractionDesc& d = nd.plan.d;
+    nd.solo = (double)d.M * (double)d.N * (double)d.K * (double)d.B >= kSoloPoints;
+    if (!nd.solo) plan_pair(h, nd.plan, j.htab, MUSCLE_HIP_TILE_AUTO, kTickSingleMax);
+  }
+  j.dtab_bytes = j.htab.empty() ? 0 : al256(j.htab.size() * sizeof(int64_t));
+}
+
+void Server::plan_loop() {
+  for (;;) {
+    std::shared_ptr<Job> j;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_plan.wait(lk, [&] { return stop || !to_compile.empty(); });
+      if (to_compile.empty()) return;  // stopping
+      j = to_compile.front();
+      to_compile.pop_front();
+    }
+    int state = kCompiled, status = MUSCLE_HIP_OK;
+    std::string msg;
+    try {
+      compile(*j);
+    } catch (const Error& e) {
+      state = kFailed;
+      status = e.status;
+      msg = e.what();
+    } catch (const std::exception& e) {
+      state = kFailed;
+      status = MUSCLE_HIP_INVALID_ARGUMENT;
+      msg = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      j->compile = state;
+      if (state == kFailed) {
+        j->status = status;
+        j->error = msg;
+      } else {
+        j->reserve += j->dtab_bytes;
+      }
+    }
+    cv_work.notify_one();
+  }
+}
+
+// Job j ends with status; its table arena goes back to the pool.
+void Server::finish_job(Job& j, int status) {
+  j.status = status;
+  j.t_done = now();
+  if (j.dtab) {
+    pool.release(j.dtab, j.dtab_bytes);
+    j.dtab = nullptr;
+    j.reserved -= j.dtab_bytes;
+    pool.reserved -= j.dtab_bytes;
+  }
+  if (status == MUSCLE_HIP_OK) st.completed++;
+}
+
 // Node n of j finished: release operands that have no consumer left.
 void Server::finish_node(Job& j, int n) {
   Node& nd = j.nodes[(size_t)n];
   nd.done = true;
   const int nl = (int)j.leaves.size();
   for (int k : {nd.lhs, nd.rhs}) {
-    if (k < nl) continue;
-    Node& src = j.nodes[(size_t)(k - nl)];
-    if (--src.consumers == 0 && !src.freed) {
-      pool.release((char*)src.out.t.data, src.bytes);
-      src.freed = true;
-      const size_t r = al256(src.bytes);
-      j.reserved -= r;
-      pool.reserved -= r;
+    if (k >= nl) {
+      Node& src = j.nodes[(size_t)(k - nl)];
+      if (--src.consumers == 0 && !src.freed) {
+        pool.release((char*)src.out.t.data, src.bytes);
+        src.freed = true;
+        const size_t r = al256(src.bytes);
+        j.reserved -= r;
+        pool.reserved -= r;
+      }
     }
     if (nd.lhs == nd.rhs) break;
   }
-  if (--j.remaining == 0) {
-    j.status = MUSCLE_HIP_OK;
-    j.t_done = now();
-    st.completed++;
-  }
+  if (--j.remaining == 0) finish_job(j, MUSCLE_HIP_OK);
 }
 
-// Admits waiting jobs in arrival order while their footprint fits.
-bool Server::admit_locked() {
-  bool any = false;
+// Admits compiled jobs in arrival order while their footprint fits; returns
+// the admitted jobs (their tables are uploaded by the caller).
+std::vector<std::shared_ptr<Job>> Server::admit_locked() {
+  std::vector<std::shared_ptr<Job>> out;
   const int cap = mode == 1 ? 1 : max_active;  // the plain loop runs one job at a time
   while (!waiting.empty() && (int)active.size() < cap) {
     std::shared_ptr<Job> j = waiting.front();
+    if (j->compile == kQueued) break;
+    if (j->compile == kFailed || j->reserve > pool.size()) {  // a contraction the library cannot run
+      if (j->compile != kFailed) {
+        j->error = "job and its tables need more device memory than the server pool holds";
+        st.rejected++;
+      }
+      waiting.pop_front();
+      j->t_admit = now();
+      finish_job(*j, j->compile == kFailed ? j->status : MUSCLE_HIP_REJECTED);
+      cv_done.notify_all();
+      continue;
+    }
     if (pool.reserved + j->reserve > pool.size()) break;
+    if (j->dtab_bytes) {
+      j->dtab = pool.alloc(j->dtab_bytes);
+      if (!j->dtab) break;  // fragmentation: retry after frees
+    }
     pool.reserved += j->reserve;
     j->reserved = j->reserve;
     j->t_admit = now();
+    for (size_t n = 0; n < j->nodes.size(); ++n)
+      if (j->nodes[n].pending == 0) j->ready.push_back((int)n);
     active.push_back(j);
     waiting.pop_front();
-    any = true;
+    out.push_back(j);
   }
-  return any;
+  return out;
 }
 
 // Launches every ready contraction of the admitted jobs as one tick.
 bool Server::launch_tick() {
   if (slot_count == kSlots) return false;
   std::vector<TickEntry> entries;
-  std::unique_lock<std::mutex> lk(mu);  // the pool and job state are shared with submit/release
-  for (auto& jp : active) {
-    Job& j = *jp;
-    const int nl = (int)j.leaves.size();
-    for (size_t n = 0; n < j.nodes.size(); ++n) {
-      Node& nd = j.nodes[n];
-      if (nd.launched) continue;
-      auto ready = [&](int k) { return k < nl || j.nodes[(size_t)(k - nl)].launched; };
-      if (!ready(nd.lhs) || !ready(nd.rhs)) continue;
-      char* p = pool.alloc(nd.bytes);
-      if (!p) continue;  // fragmentation: retry after frees
-      nd.out.t.data = p;
-      entries.push_back({&j, (int)n});
+  {
+    std::lock_guard<std::mutex> lk(mu);  // the pool and job state are shared with submit/release
+    for (auto& jp : active) {
+      Job& j = *jp;
+      if (j.ready.empty()) continue;
+      std::vector<int> cand;
+      cand.swap(j.ready);
+      const size_t first = entries.size();
+      for (int n : cand) {
+        Node& nd = j.nodes[(size_t)n];
+        char* p = pool.alloc(nd.bytes);
+        if (!p) {  // fragmentation: retry after frees
+          j.ready.push_back(n);
+          continue;
+        }
+        nd.out.t.data = p;
+        entries.push_back({&j, n});
+      }
+      // consumers become ready for the next tick (not this one: a batched
+      // launch has no order between its contractions)
+      for (size_t e = first; e < entries.size(); ++e) {
+        Node& nd = j.nodes[(size_t)entries[e].node];
+        nd.launched = true;
+        for (int u : nd.users)
+          if (--j.nodes[(size_t)u].pending == 0) j.ready.push_back(u);
+      }
+      if (entries.size() > first) j.ticks++;
     }
-    // mark after the scan so a node launched now is not an operand of another node in this tick
-    for (auto& e : entries)
-      if (e.job == &j) j.nodes[(size_t)e.node].launched = true;
   }
-  for (auto& e : entries) e.job->ticks++;
-  lk.unlock();
   if (entries.empty()) return false;
   Slot& sl = slots[(slot_head + slot_count) % kSlots];
   sl.entries = entries;
   sl.batches.clear();
-  // Contractions big enough to fill the GPU alone take the single-contraction
-  // kernel (deeper pipelining, split-K); the rest share one persistent launch.
-  std::vector<char> solo(entries.size(), 0);
-  for (size_t i = 0; i < entries.size(); ++i) {
-    Job& j = *entries[i].job;
-    const Node& nd = j.nodes[(size_t)entries[i].node];
-    if (pair_points(j.operand(nd.lhs), j.operand(nd.rhs)) < kSoloPoints) continue;
-    solo[i] = 1;
+  size_t used = 0;
+  std::vector<const PlannedPair*> lw[MUSCLE_HIP_C64 + 1];
+  for (const TickEntry& e : entries) {
+    Job& j = *e.job;
+    Node& nd = j.nodes[(size_t)e.node];
     const muscle_hip_tensor_t& a = j.operand(nd.lhs).t;
     const muscle_hip_tensor_t& b = j.operand(nd.rhs).t;
-    Plan* pl = get_plan(h, 1, MUSCLE_HIP_PRECISION_NATIVE, 1e-12, &nd.out.t, &a, &b);
-    run_contraction(h, *pl, nd.out.t, a, &b);
-    st.launched_pairs++;
-  }
-  size_t used = 0;
-  for (int dt = MUSCLE_HIP_F32; dt <= MUSCLE_HIP_C64; ++dt) {
-    std::vector<muscle_hip_tensor_t> cs, as, bs;
-    for (size_t i = 0; i < entries.size(); ++i) {
-      const TickEntry& e = entries[i];
-      Job& j = *e.job;
-      if (j.dtype != dt || solo[i]) continue;
-      const Node& nd = j.nodes[(size_t)e.node];
-      cs.push_back(nd.out.t);
-      as.push_back(j.operand(nd.lhs).t);
-      bs.push_back(j.operand(nd.rhs).t);
+    if (nd.solo) {
+      Plan* pl = get_plan(h, 1, MUSCLE_HIP_PRECISION_NATIVE, 1e-12, &nd.out.t, &a, &b);
+      run_contraction(h, *pl, nd.out.t, a, &b);
+      st.launched_pairs++;
+      continue;
     }
-    if (cs.empty()) continue;
+    PlannedPair& L = nd.plan;
+    L.a = L.d.swapped ? b.data : a.data;
+    L.b = L.d.swapped ? a.data : b.data;
+    L.c = nd.out.t.data;
+    L.dtab = (const int64_t*)j.dtab;
+    lw[j.dtype].push_back(&L);
+  }
+  for (int dt = MUSCLE_HIP_F32; dt <= MUSCLE_HIP_C64; ++dt) {
+    if (lw[dt].empty()) continue;
     std::unique_ptr<Batch> B(new Batch());
-    std::vector<PlannedPair> lw;
-    plan_batch(h, (int64_t)cs.size(), cs.data(), as.data(), bs.data(), nullptr, false, *B, lw);
-    const BatchLayout l = batch_layout(*B, (int64_t)cs.size());
+    B->h = &h;
+    B->dtype = dt;
+    finish_batch(h, *B, lw[dt]);
+    const BatchLayout l = batch_layout(*B, (int64_t)lw[dt].size());
     if (used + l.total <= sl.size) {
-      place_batch(*B, lw, sl.dev + used, sl.pinned + used, h.stream);
+      place_batch(*B, lw[dt], sl.dev + used, sl.pinned + used, h.stream);
       used += al256(l.total);
     } else {
       void* d = nullptr;
       if (hipMalloc(&d, l.total) != hipSuccess) fail(MUSCLE_HIP_OUT_OF_MEMORY, "tick descriptors");
       sl.extra_dev.push_back(d);
-      place_batch(*B, lw, (char*)d, nullptr, h.stream);
+      place_batch(*B, lw[dt], (char*)d, nullptr, h.stream);
     }
-    batch_launch(*B, h.stream);
+    if (!B->hitems.empty()) batch_launch(*B, h.stream);
     MH_HIP_CHECK(hipGetLastError());
     B->h = nullptr;  // completion is tracked by the slot event
-    st.launched_pairs += (int64_t)cs.size();
+    st.launched_pairs += (int64_t)lw[dt].size();
     sl.batches.push_back(std::move(B));
   }
   MH_HIP_CHECK(hipEventRecord(sl.ev, h.stream));
-  sl.busy = true;
   slot_count++;
   st.ticks++;
   return true;
@@ -354,7 +445,6 @@ bool Server::complete_ticks(bool block) {
     sl.extra_dev.clear();
     sl.batches.clear();
     sl.entries.clear();
-    sl.busy = false;
     slot_head = (slot_head + 1) % kSlots;
     slot_count--;
     any = true;
@@ -364,8 +454,6 @@ bool Server::complete_ticks(bool block) {
 
 // The plain per-call loop: one launch per node, one job at a time.
 void Server::run_serial(Job& j) {
-  const int nl = (int)j.leaves.size();
-  (void)nl;
   for (size_t n = 0; n < j.nodes.size(); ++n) {
     Node& nd = j.nodes[n];
     char* p;
@@ -397,19 +485,30 @@ void Server::loop() {
   try {
     for (;;) {
       bool progress = false;
+      std::vector<std::shared_ptr<Job>> admitted;
       {
         std::unique_lock<std::mutex> lk(mu);
-        if (stop && waiting.empty() && active.empty() && slot_count == 0) break;
-        if (mode == 1) {
-          if (active.empty()) progress |= admit_locked();
-        } else {
-          progress |= admit_locked();
+        if (stop && active.empty() && slot_count == 0) {
+          for (auto& j : waiting) {  // never admitted: ended by the shutdown
+            j->error = "server destroyed before the job was admitted";
+            j->status = MUSCLE_HIP_HIP_ERROR;
+          }
+          waiting.clear();
+          cv_done.notify_all();
+          break;
         }
-        if (!progress && waiting.empty() && active.empty() && slot_count == 0) {
+        if (!stop && (mode == 0 || active.empty())) admitted = admit_locked();
+        progress = !admitted.empty();
+        if (!progress && active.empty() && slot_count == 0) {
+          // nothing admitted and nothing running: wait for a submission or a compiled job
           cv_work.wait_for(lk, std::chrono::milliseconds(20));
           continue;
         }
       }
+      for (auto& j : admitted)
+        if (j->dtab_bytes)
+          MH_HIP_CHECK(hipMemcpyAsync(j->dtab, j->htab.data(), j->htab.size() * sizeof(int64_t),
+                                      hipMemcpyHostToDevice, h.stream));
       if (mode == 1) {
         std::shared_ptr<Job> j;
         {
@@ -423,15 +522,25 @@ void Server::loop() {
       progress |= complete_ticks(false);
       progress |= launch_tick();
       if (!progress) {
-        if (slot_count > 0) complete_ticks(true);  // nothing to launch: wait for the GPU
-        else std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (slot_count > 0) {
+          complete_ticks(true);  // nothing to launch: wait for the GPU
+        } else {
+          std::unique_lock<std::mutex> lk(mu);  // admitted jobs wait for memory: wait for a release
+          cv_work.wait_for(lk, std::chrono::microseconds(200));
+        }
       }
     }
   } catch (const std::exception& e) {
     std::lock_guard<std::mutex> lk(mu);
     worker_error = e.what();
-    for (auto& j : active) j->status = MUSCLE_HIP_HIP_ERROR;
-    for (auto& j : waiting) j->status = MUSCLE_HIP_HIP_ERROR;
+    for (auto& j : active) {
+      j->status = MUSCLE_HIP_HIP_ERROR;
+      j->error = worker_error;
+    }
+    for (auto& j : waiting) {
+      j->status = MUSCLE_HIP_HIP_ERROR;
+      j->error = worker_error;
+    }
     active.clear();
     waiting.clear();
     cv_done.notify_all();
@@ -495,6 +604,8 @@ int muscle_hip_server_create(int64_t pool_bytes, int max_active_jobs, int mode, 
     S->st.pool_bytes = pool_bytes;
     Server* raw = S.get();
     S->worker = std::thread([raw] { raw->loop(); });
+    const int np = std::max(1, std::min<int>(kMaxPlanners, (int)std::thread::hardware_concurrency() / 2));
+    for (int i = 0; i < np; ++i) S->planners.emplace_back([raw] { raw->plan_loop(); });
     *server = reinterpret_cast<muscle_hip_server_t>(S.release());
   });
 }
@@ -542,32 +653,35 @@ int muscle_hip_server_submit(muscle_hip_server_t server, const muscle_hip_job_t*
     for (int i = 0; i < job->n_nodes; ++i) {
       Node& nd = j->nodes[(size_t)i];
       for (int k : {nd.lhs, nd.rhs}) {
-        if (k >= job->n_leaves) j->nodes[(size_t)(k - job->n_leaves)].consumers++;
+        if (k >= job->n_leaves) {
+          Node& src = j->nodes[(size_t)(k - job->n_leaves)];
+          src.consumers++;
+          src.users.push_back(i);
+          nd.pending++;
+        }
         if (nd.lhs == nd.rhs) break;
       }
     }
     // the root is consumed by the caller (muscle_hip_server_release)
     j->nodes.back().consumers++;
     j->remaining = job->n_nodes;
-    // lower every node now so malformed contractions fail at submission
-    for (int i = 0; i < job->n_nodes; ++i) {
-      Node& nd = j->nodes[(size_t)i];
-      muscle_hip_tensor_t c = nd.out.t;
-      c.data = (void*)256;  // placeholder address for validation only
-      (void)lower_binary(c, j->operand(nd.lhs).t, j->operand(nd.rhs).t);
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->st.submitted++;
+      if (j->reserve > S->pool.size()) {
+        S->st.rejected++;
+        fail(MUSCLE_HIP_REJECTED, "job needs more device memory than the server pool holds");
+      }
+      j->id = S->next_id++;
+      j->t_submit = S->now();
+      S->jobs[j->id] = j;
+      S->waiting.push_back(j);
+      S->to_compile.push_back(j);
+      *job_id = j->id;
     }
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->st.submitted++;
-    if (j->reserve > S->pool.size()) {
-      S->st.rejected++;
-      fail(MUSCLE_HIP_REJECTED, "job needs more device memory than the server pool holds");
-    }
-    j->id = S->next_id++;
-    j->t_submit = S->now();
-    S->jobs[j->id] = j;
-    S->waiting.push_back(j);
-    *job_id = j->id;
-    S->cv_work.notify_one();
+    // contractions are lowered and planned on the planner threads; one the
+    // library cannot run ends the job with that status (seen by wait)
+    S->cv_plan.notify_one();
   });
 }
 
@@ -597,7 +711,7 @@ int muscle_hip_server_wait(muscle_hip_server_t server, int64_t job_id, double ti
     result->ticks = j->ticks;
   }
   if (j->status == MUSCLE_HIP_NOT_READY) return MUSCLE_HIP_NOT_READY;
-  if (j->status != MUSCLE_HIP_OK) g_last_error = S->worker_error;
+  if (j->status != MUSCLE_HIP_OK) g_last_error = j->error;
   return j->status;
 }
 
@@ -668,6 +782,8 @@ int muscle_hip_server_destroy(muscle_hip_server_t server) {
       S->stop = true;
     }
     S->cv_work.notify_all();
+    S->cv_plan.notify_all();
+    for (auto& t : S->planners) t.join();
     if (S->worker.joinable()) S->worker.join();
     (void)hipStreamSynchronize(S->h.stream);
     for (Slot& sl : S->slots) {

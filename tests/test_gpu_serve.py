@@ -44,10 +44,8 @@ def job_bytes(spec):
 
 def run_jobs(srv, bank, specs, seed=0):
     rng = np.random.default_rng(seed)
-    jobs = []
-    for spec in specs:
-        leaves = bank.leaves(spec, rng)
-        jobs.append((spec, leaves, srv.submit(leaves, tree_nodes(spec))))
+    prepared = [(spec, srv.prepare(bank.leaves(spec, rng), tree_nodes(spec))) for spec in specs]
+    jobs = [(spec, pj.leaves, srv.submit_prepared(pj)) for spec, pj in prepared]
     out = []
     for spec, leaves, jid in jobs:
         res = srv.wait(jid, timeout=120.0)
@@ -76,16 +74,15 @@ def test_served_jobs_match_oracle():
 def test_alone_equals_in_mix(dtype):
     bank = LeafBank(seed=2)
     target = small_trees([200], dtype=dtype, n_nodes=(40, 40), size_log2=(4, 12))[0]
-    others = small_trees(range(201, 213), size_log2=(4, 12))
+    others = small_trees(range(201, 213), n_nodes=(100, 200), size_log2=(8, 16))  # a few ms each
     srv = Server(pool_gib=4)
     (_, leaves, alone, _), = run_jobs(srv, bank, [target], seed=5)
     # the same job (same leaf tensors) submitted in the middle of 12 others
     mix = others[:6] + [target] + others[6:]
-    jobs = []
     rng = np.random.default_rng(9)
-    for spec in mix:
-        lv = leaves if spec is target else bank.leaves(spec, rng)
-        jobs.append(srv.submit(lv, tree_nodes(spec)))
+    prepared = [srv.prepare(leaves if spec is target else bank.leaves(spec, rng), tree_nodes(spec))
+                for spec in mix]
+    jobs = [srv.submit_prepared(pj) for pj in prepared]  # back to back, so the jobs share ticks
     results, spans = [], []
     for jid in jobs:
         r = srv.wait(jid, timeout=120.0)
@@ -137,10 +134,17 @@ def test_admission_queues_jobs_until_memory_frees():
         assert rel_err(host(got), oracle_root(spec, leaves)) <= 1e-12
     st = srv.stats()
     assert st["completed"] == len(specs) and st["pool_peak_bytes"] <= pool
-    # a job admitted only after an earlier one had finished
-    admits = sorted((r["admit_s"], r["done_s"]) for _, _, _, r in out)
-    assert any(a2 >= d1 for (a1, d1), (a2, d2) in zip(admits, admits[1:]))
+    # the jobs were submitted back to back but never all admitted together
+    spans = [(r["admit_s"], r["done_s"]) for _, _, _, r in out]
+    most = max(sum(1 for a, d in spans if a <= t < d) for t, _ in spans)
+    assert most < len(specs), spans
+    assert max(r["admit_s"] - r["submit_s"] for _, _, _, r in out) > 0
     srv.close()
+
+
+def operand_inds(spec, leaves, k):
+    nl = len(leaves)
+    return leaves[k].inds() if k < nl else [muscle.Index(i) for i in spec.nodes[k - nl][2]]
 
 
 def test_invalid_jobs():
@@ -160,6 +164,16 @@ def test_invalid_jobs():
     mixed = [muscle.Tensor(t.parent().to(torch.complex64), t.inds()) if k == 0 else t for k, t in enumerate(leaves)]
     with pytest.raises(muscle.ArgumentError):
         srv.submit(mixed, nodes)
+    # an output index that neither operand has: found when the job is planned,
+    # reported by wait
+    bad = list(nodes)
+    foreign = next(i for t in leaves for i in t.inds()
+                   if i not in operand_inds(spec, leaves, bad[0][0]) + operand_inds(spec, leaves, bad[0][1]))
+    bad[0] = (bad[0][0], bad[0][1], list(bad[0][2]) + [foreign])
+    jid = srv.submit(leaves, bad)
+    with pytest.raises(muscle.ArgumentError):
+        srv.wait(jid, timeout=120.0)
+    srv.release(jid)
     jid = srv.submit(leaves, nodes)  # the server still works
     assert srv.wait(jid, timeout=120.0)["status"] == 0
     srv.release(jid)
