@@ -1,0 +1,172 @@
+"""
+    MuscleHIPExt
+
+Muscle.jl package extension that runs the contraction hot path on an AMD
+MI355X through `libmuscle_hip.so` (a plain C ABI, `include/muscle_hip.h`).
+
+It plugs in where `ext/MuscleCUDAExt.jl` plugs in for CUDA: a Domain for
+`ROCArray`, rules that select `BackendHIP` for device operands, and the six
+backend methods the front ends call:
+
+  binary_einsum(::BackendHIP, inds_c, a, b)   (call site src/Operations/binary_einsum.jl:50)
+  binary_einsum!(::BackendHIP, c, a, b)       (:68)
+  hadamard(::BackendHIP, a, b)                (src/Operations/hadamard.jl:12)
+  hadamard!(::BackendHIP, c, a, b)            (:27)
+  unary_einsum(::BackendHIP, inds_y, x)       (src/Operations/unary_einsum.jl:35)
+  unary_einsum!(::BackendHIP, y, x)           (:44)
+
+It assumes `struct BackendHIP <: Backend end` next to the other backends
+(src/Backend.jl:6-14) and `struct DomainROCm <: Domain end` next to the other
+domains (src/Domain.jl:6-9).  Set ENV["MUSCLE_HIP_LIB"] to the library path.
+"""
+module MuscleHIPExt
+
+using Muscle
+using Muscle: Tensor, Index, inds, BackendHIP, DomainROCm
+using AMDGPU
+
+const LIB = get(ENV, "MUSCLE_HIP_LIB", "libmuscle_hip")
+
+# --- C ABI mirror ------------------------------------------------------------
+
+const DTYPE = Dict{DataType,Int32}(Float32 => 0, Float64 => 1, ComplexF32 => 2, ComplexF64 => 3)
+const STATUS_OK, STATUS_INVALID, STATUS_DIM, STATUS_UNSUPPORTED = 0, 1, 2, 3
+
+# muscle_hip_tensor_t
+struct CTensor
+    dtype::Int32
+    rank::Int32
+    data::Ptr{Cvoid}
+    modes::Ptr{Int32}
+    extents::Ptr{Int64}
+    strides::Ptr{Int64}
+end
+
+# muscle_hip_info_t (read back for introspection)
+struct CInfo
+    kernel::Int32
+    precision::Int32
+    slices::Int32
+    swapped::Int32
+    m::Int64
+    n::Int64
+    k::Int64
+    batch::Int64
+    flops::Float64
+    bytes::Float64
+    err_bound::Float64
+end
+
+function check(status::Integer)
+    status == STATUS_OK && return nothing
+    msg = unsafe_string(ccall((:muscle_hip_last_error, LIB), Cstring, ()))
+    status == STATUS_DIM && throw(DimensionMismatch(msg))
+    status in (STATUS_INVALID, STATUS_UNSUPPORTED) && throw(ArgumentError(msg))
+    name = unsafe_string(ccall((:muscle_hip_status_string, LIB), Cstring, (Cint,), status))
+    error("libmuscle_hip: $name: $msg")
+end
+
+# One library handle per task and device; the stream is AMDGPU's task-local
+# stream, set before every call (binary_einsum's backend override is
+# task-scoped too, src/Backend.jl:16-18).
+function handle()
+    key = (:muscle_hip_handle, AMDGPU.device_id(AMDGPU.device()))
+    h = get(task_local_storage(), key, C_NULL)
+    if h == C_NULL
+        ref = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:muscle_hip_create, LIB), Cint, (Ref{Ptr{Cvoid}},), ref))
+        h = ref[]
+        task_local_storage(key, h)
+    end
+    check(ccall((:muscle_hip_set_stream, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), h, AMDGPU.stream().stream))
+    return h
+end
+
+# Mode labels: Dict(ind => i for (i, ind) in enumerate(unique(...))), as
+# ext/MuscleCUDAExt.jl:24-27 builds them.
+modemap(tensors...) = Dict{Index,Int32}(ind => Int32(i) for (i, ind) in enumerate(unique(vcat(map(inds, tensors)...))))
+
+# A C descriptor of `t` with modes `labels`; the returned arrays must be kept
+# alive (GC.@preserve) across the ccall.
+function describe(t::Tensor, labels::Dict{Index,Int32})
+    arr = parent(t)
+    modes = Int32[labels[i] for i in inds(t)]
+    exts = Int64[size(arr)...]
+    strs = Int64[strides(arr)...]
+    desc = CTensor(DTYPE[eltype(arr)], Int32(ndims(arr)), Ptr{Cvoid}(pointer(arr)),
+                   pointer(modes), pointer(exts), pointer(strs))
+    return desc, (modes, exts, strs)
+end
+
+function contract!(c::Tensor, a::Tensor, b::Tensor; precision::Integer=0, target::Real=1e-12)
+    labels = modemap(a, b, c)
+    dc, kc = describe(c, labels)
+    da, ka = describe(a, labels)
+    db, kb = describe(b, labels)
+    GC.@preserve c a b kc ka kb begin
+        check(ccall((:muscle_hip_binary_einsum, LIB), Cint,
+                    (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Cint, Cdouble, Ptr{CInfo}),
+                    handle(), dc, da, db, precision, target, C_NULL))
+    end
+    return c
+end
+
+# --- dispatch ------------------------------------------------------------------
+
+Muscle.Domain(::Type{<:ROCArray}) = DomainROCm()
+
+Muscle.choose_backend_rule(::typeof(binary_einsum), ::DomainROCm, ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(binary_einsum!), ::DomainROCm, ::DomainROCm, ::DomainROCm) = BackendHIP()
+# more specific than the generic `::Domain` rules of src/Operations/hadamard.jl:4-5
+Muscle.choose_backend_rule(::typeof(hadamard), ::DomainROCm, ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(hadamard!), ::DomainROCm, ::DomainROCm, ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(unary_einsum), ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(unary_einsum!), ::DomainROCm, ::DomainROCm) = BackendHIP()
+
+function Muscle.binary_einsum(::BackendHIP, inds_c, a::Tensor, b::Tensor)
+    sizes = Dict{Index,Int}()
+    for t in (a, b), (i, n) in zip(inds(t), size(parent(t)))
+        sizes[i] = n
+    end
+    T = Base.promote_eltype(parent(a), parent(b))
+    c = Tensor(similar(parent(a), T, Tuple(sizes[i] for i in inds_c)), collect(inds_c))
+    return contract!(c, a, b)
+end
+
+Muscle.binary_einsum!(::BackendHIP, c::Tensor, a::Tensor, b::Tensor) = contract!(c, a, b)
+
+function Muscle.hadamard!(::BackendHIP, c::Tensor, a::Tensor, b::Tensor)
+    labels = modemap(a, b)
+    dc, kc = describe(c, labels)
+    da, ka = describe(a, labels)
+    db, kb = describe(b, labels)
+    GC.@preserve c a b kc ka kb begin
+        check(ccall((:muscle_hip_hadamard, LIB), Cint,
+                    (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Ptr{CInfo}),
+                    handle(), dc, da, db, C_NULL))
+    end
+    return c
+end
+
+# the result has a's indices and a's element type (copy(a), src/Operations/hadamard.jl:38)
+Muscle.hadamard(::BackendHIP, a::Tensor, b::Tensor) =
+    Muscle.hadamard!(BackendHIP(), Tensor(similar(parent(a)), inds(a)), a, b)
+
+function Muscle.unary_einsum!(::BackendHIP, y::Tensor, x::Tensor)
+    labels = modemap(x)
+    dy, ky = describe(y, labels)
+    dx, kx = describe(x, labels)
+    GC.@preserve y x ky kx begin
+        check(ccall((:muscle_hip_unary_einsum, LIB), Cint, (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ptr{CInfo}),
+                    handle(), dy, dx, C_NULL))
+    end
+    return y
+end
+
+function Muscle.unary_einsum(::BackendHIP, inds_y, x::Tensor)
+    sizes = Dict(zip(inds(x), size(parent(x))))
+    y = Tensor(similar(parent(x), Tuple(sizes[i] for i in inds_y)), collect(inds_y))
+    return Muscle.unary_einsum!(BackendHIP(), y, x)
+end
+
+end # module

@@ -9,7 +9,6 @@ this package, as the checker and as the timed CPU baseline; the product
                 unary (OMEinsum) rule and hadamard.
   brute         long-double brute-force einsum in C (oracle/brute.c), the
                 independent check that pins backend_base.
-  gen           seeded input generators for the workloads.
 
 The reference (Julia) cannot be built or run here: no Julia toolchain is
 installed, so there is no oracle/_ref build.  Every oracle function is pinned
