@@ -76,9 +76,16 @@ void Handle::ensure_workspace(size_t bytes) {
   ws_size = want;
 }
 
+Plan::~Plan() {
+  if (dtab) (void)hipFree(dtab);
+  if (split_dev) (void)hipFree(split_dev);
+  release_split_lib(*this);
+}
+
 Handle::~Handle() {
   if (ws) (void)hipFree(ws);
   cache.clear();  // plans free their device tables
+  release_lt(*this);
 }
 
 Plan* Handle::find_plan(const std::string& key) {
@@ -239,7 +246,7 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
   if (!info) return;
   std::memset(info, 0, sizeof *info);
   const ContractionDesc& d = p.d;
-  info->kernel = p.kind == Plan::GEMM ? 1 : p.kind == Plan::REDUCE ? 2 : p.kind == Plan::SPLIT ? 3 : 0;
+  info->kernel = p.kind == Plan::GEMM ? 1 : p.kind == Plan::REDUCE ? 2 : p.kind == Plan::SPLIT ? (p.split_lib ? 5 : 3) : 0;
   info->precision = p.fp32_math ? MUSCLE_HIP_PRECISION_FP32
                                  : (p.kind == Plan::SPLIT ? p.precision : MUSCLE_HIP_PRECISION_NATIVE);
   info->slices = p.slices;

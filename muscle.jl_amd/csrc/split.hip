@@ -12,7 +12,11 @@
 //               re, im and re + im (3M / Gauss products).
 //   3. GEMM:    for every group of slice pairs (same weight), a dense int8 or
 //               bf16/f16 MFMA GEMM over the concatenated K segments; int32
-//               sums are exact while K * pairs * 127^2 < 2^31.
+//               sums are exact while K * pairs * 127^2 < 2^31.  A packed row
+//               holds the S slices of one operand row side by side (B's in
+//               reverse order), so the pairs of weight d, (i, d - i), are ONE
+//               contiguous K window of A times one of B: large int8 plans run
+//               each group as a plain strided-batched library GEMM.
 //   4. combine: C = 2^(eA + eB) * sum over groups of weight * group sum, in
 //               FP64, scattered to C's own strided layout.
 //
@@ -21,8 +25,11 @@
 // emulation (2024); digits are balanced base-255 (-127..127) so each slice
 // carries log2(255) bits.  BF16x3 / FP16x2 are the hi/mid/lo splits of
 // Henry, Tang & Heinecke (2019).
+#include <hipblaslt/hipblaslt.h>
+
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "device_types.hpp"
 #include "kernels.hpp"
@@ -70,9 +77,10 @@ struct PackArgs {
   const void* X;
   OffTab rowX, kX, bX;
   const int32_t* exps;  // [B][R] or null (no scaling)
-  void* Q;              // [planes*S][B][Rp][Kp] elements
+  void* Q;              // [planes][B][Rp][S][Kp] elements
   int32_t R, K, B, Rp, Kp;
   int32_t cplx, r_major, planes, S, mode;
+  int32_t rev;          // slices stored in reverse order within a row
 };
 
 constexpr int PR = 32, PK = 64;  // pack tile: 32 rows x 64 k
@@ -136,13 +144,16 @@ __global__ __launch_bounds__(NT) void pack_kernel(PackArgs p) {
   // write out 16-byte chunks of K-contiguous rows
   constexpr int CPR = PK * ES / 16;  // chunks per row
   const int total = nps * PR * CPR;
-  const int64_t rowbytes = (int64_t)p.Kp * ES;
+  const int64_t slot = (int64_t)p.Kp * ES, rowbytes = slot * p.S;
   for (int c = threadIdx.x; c < total; c += NT) {
     const int ch = c % CPR;
     const int rl = (c / CPR) % PR;
     const int ps = c / (CPR * PR);
+    const int pl = ps / p.S, sl = ps - pl * p.S;
+    const int pos = p.rev ? p.S - 1 - sl : sl;
     const uint4 val = *reinterpret_cast<const uint4*>(&tile[ps][rl][ch * 16]);
-    uint8_t* out = (uint8_t*)p.Q + (((int64_t)ps * p.B + b) * p.Rp + r0 + rl) * rowbytes + (int64_t)k0 * ES + ch * 16;
+    uint8_t* out = (uint8_t*)p.Q + (((int64_t)pl * p.B + b) * p.Rp + r0 + rl) * rowbytes + pos * slot +
+                   (int64_t)k0 * ES + ch * 16;
     *reinterpret_cast<uint4*>(out) = val;
   }
 }
@@ -151,14 +162,22 @@ __global__ __launch_bounds__(NT) void pack_kernel(PackArgs p) {
 struct SlicedArgs {
   const uint8_t* QA;
   const uint8_t* QB;
-  int64_t a_ps_stride, b_ps_stride;  // bytes between plane-slices
-  int64_t a_b_stride, b_b_stride;    // bytes between batch entries
-  int32_t rowbytes;
+  int64_t a_plane_stride, b_plane_stride;  // bytes between planes
+  int64_t a_b_stride, b_b_stride;          // bytes between batch entries
+  int64_t rowbytes;                        // bytes per packed row (S slices)
+  int32_t slot_bytes;                      // bytes per slice within a row
+  int32_t S;
   const SliceSeg* segs;
   const SliceGroup* groups;
   void* out;  // [G][B][Mp][Np], int32 or float
   int32_t Mp, Np, B, tiles_m, tiles_n, esize;
 };
+
+// Byte offset of slot (plane * S + position) in a packed operand.
+__device__ __forceinline__ int64_t slot_offset(int slot, int S, int64_t plane_stride, int slot_bytes) {
+  const int pl = slot / S;
+  return pl * plane_stride + (int64_t)(slot - pl * S) * slot_bytes;
+}
 
 constexpr int SBM = 128, SBN = 128, SKB = 128;  // tile rows/cols, K bytes per stage
 constexpr int SROW = SKB + 16;                  // padded LDS row: 16-lane groups hit disjoint banks
@@ -231,11 +250,14 @@ __global__ __launch_bounds__(NT, 2) void sliced_gemm_kernel(SlicedArgs p) {
   int cs = 0;
   SliceSeg csg = p.segs[grp.first];
   int ckb = csg.k0 * p.esize;
+  // segment slots a, b = plane * S + position of the slice within a row
+  int64_t aseg = slot_offset(csg.a, p.S, p.a_plane_stride, p.slot_bytes);
+  int64_t bseg = slot_offset(csg.b, p.S, p.b_plane_stride, p.slot_bytes);
   uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
 #define MH_FETCH()                                                            \
   do {                                                                        \
-    const uint8_t* pa = p.QA + csg.a * p.a_ps_stride + arow0 + ckb;           \
-    const uint8_t* pb = p.QB + csg.b * p.b_ps_stride + brow0 + ckb;           \
+    const uint8_t* pa = p.QA + aseg + arow0 + ckb;                            \
+    const uint8_t* pb = p.QB + bseg + brow0 + ckb;                            \
     ra0 = *reinterpret_cast<const uint4*>(pa);                                \
     ra1 = *reinterpret_cast<const uint4*>(pa + step32);                       \
     ra2 = *reinterpret_cast<const uint4*>(pa + 2 * step32);                   \
@@ -263,6 +285,8 @@ __global__ __launch_bounds__(NT, 2) void sliced_gemm_kernel(SlicedArgs p) {
       ++cs;                                                            \
       csg = p.segs[grp.first + cs];                                    \
       ckb = csg.k0 * p.esize;                                          \
+      aseg = slot_offset(csg.a, p.S, p.a_plane_stride, p.slot_bytes);  \
+      bseg = slot_offset(csg.b, p.S, p.b_plane_stride, p.slot_bytes);  \
     }                                                                  \
   } while (0)
 
@@ -382,11 +406,11 @@ WsLayout ws_layout(const Plan& p) {
   off += al((size_t)d.B * p.Mp * 4);
   w.expB = off;
   off += al((size_t)d.B * p.Np * 4);
-  const size_t rowbytes = (size_t)p.Kp * p.esize;
+  const size_t rowbytes = (size_t)p.Kp * p.esize * p.slices;  // S slices per packed row
   w.QA = off;
-  off += al((size_t)p.planes_a * p.slices * d.B * p.Mp * rowbytes);
+  off += al((size_t)p.planes_a * d.B * p.Mp * rowbytes);
   w.QB = off;
-  off += al((size_t)p.planes_b * p.slices * d.B * p.Np * rowbytes);
+  off += al((size_t)p.planes_b * d.B * p.Np * rowbytes);
   w.G32 = off;
   off += al((size_t)p.ngroups * d.B * p.Mp * p.Np * 4);
   w.total = off;
@@ -465,7 +489,10 @@ void prepare_split(Plan& p, double target) {
   const std::vector<Product> prods = products_for(ca, cb);
   p.split_segs.clear();
   p.split_groups.clear();
+  p.lib_groups.clear();
   std::vector<double> cre, cim;
+  // slots: A slice i (1-based) sits at position i - 1 of its row, B slice j
+  // at position S - j (reversed)
   for (const Product& pr : prods)
     for (size_t gi = 0; gi < pair_groups.size(); ++gi)
       for (int ks = 0; ks < nks; ++ks) {
@@ -474,14 +501,21 @@ void prepare_split(Plan& p, double target) {
         p.split_groups.push_back((int32_t)pair_groups[gi].size());
         for (auto& st : pair_groups[gi]) {
           p.split_segs.push_back(pr.a_plane * p.slices + st.first - 1);
-          p.split_segs.push_back(pr.b_plane * p.slices + st.second - 1);
+          p.split_segs.push_back(pr.b_plane * p.slices + p.slices - st.second);
           p.split_segs.push_back(k0);
           p.split_segs.push_back(k1);
         }
+        p.lib_groups.push_back(pr.a_plane);
+        p.lib_groups.push_back(pr.b_plane);
+        p.lib_groups.push_back((int32_t)gi + 2);  // weight d: pairs (i, d - i)
         cre.push_back(pr.sre * weights[gi]);
         cim.push_back(pr.sim * weights[gi]);
       }
   p.ngroups = (int)cre.size();
+  // int8 groups as library GEMMs: one K chunk, and large enough that 15-45
+  // launches cost nothing next to the work
+  p.split_lib = p.split_mode == MODE_I8 && nks == 1 && d.M * d.N >= (int64_t)512 * 512 &&
+                (double)d.M * (double)d.N * (double)d.K * (double)d.B >= 1e9;
   p.split_coef = cre;
   p.split_coef.insert(p.split_coef.end(), cim.begin(), cim.end());
 }
@@ -509,6 +543,180 @@ void upload_split_tables(Plan& p, hipStream_t s) {
                               hipMemcpyHostToDevice, s));
 }
 
+// ------------------------------------------------------------------ library GEMMs
+// The int8 group sums of large plans are plain dense GEMMs (K-contiguous rows,
+// int32 results), so they run on hipBLASLt: for weight d, the K window of A
+// holding slices 1..d-1 times the K window of B holding slices d-1..1.
+
+constexpr size_t kLtWorkspace = size_t(32) << 20;
+constexpr int kLtCandidates = 8;
+
+struct LtGroups {
+  hipblasLtMatmulDesc_t desc = nullptr;
+  struct Group {
+    hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
+    hipblasLtMatmulAlgo_t cand[kLtCandidates];
+    int ncand = 0, best = -1;
+  };
+  std::vector<Group> g;  // index d - 2
+};
+
+void release_split_lib(Plan& p) {
+  if (!p.lt) return;
+  for (auto& g : p.lt->g) {
+    if (g.la) (void)hipblasLtMatrixLayoutDestroy(g.la);
+    if (g.lb) (void)hipblasLtMatrixLayoutDestroy(g.lb);
+    if (g.lc) (void)hipblasLtMatrixLayoutDestroy(g.lc);
+  }
+  if (p.lt->desc) (void)hipblasLtMatmulDescDestroy(p.lt->desc);
+  delete p.lt;
+  p.lt = nullptr;
+}
+
+void release_lt(Handle& h) {
+  if (h.lt_ws) (void)hipFree(h.lt_ws);
+  if (h.lt) (void)hipblasLtDestroy((hipblasLtHandle_t)h.lt);
+  h.lt_ws = nullptr;
+  h.lt = nullptr;
+}
+
+namespace {
+
+#define MH_LT_OK(x) ((x) == HIPBLAS_STATUS_SUCCESS)
+
+// Builds descriptors, layouts and candidate algorithms; false when the
+// library has no algorithm for some group (the plan then keeps the own kernel).
+bool setup_lt(Handle& h, Plan& p, const SlicedArgs& ga) {
+  if (!h.lt) {
+    hipblasLtHandle_t lt = nullptr;
+    if (!MH_LT_OK(hipblasLtCreate(&lt))) return false;
+    h.lt = lt;
+    if (hipMalloc(&h.lt_ws, kLtWorkspace) != hipSuccess) {
+      (void)hipGetLastError();
+      release_lt(h);
+      return false;
+    }
+  }
+  const ContractionDesc& d = p.d;
+  std::unique_ptr<LtGroups> L(new LtGroups());
+  auto cleanup = [&] {
+    p.lt = L.release();
+    release_split_lib(p);
+    return false;
+  };
+  if (!MH_LT_OK(hipblasLtMatmulDescCreate(&L->desc, HIPBLAS_COMPUTE_32I, HIP_R_32I))) return cleanup();
+  const hipblasOperation_t ta = HIPBLAS_OP_T, tb = HIPBLAS_OP_N;
+  if (!MH_LT_OK(hipblasLtMatmulDescSetAttribute(L->desc, HIPBLASLT_MATMUL_DESC_TRANSA, &ta, sizeof ta)) ||
+      !MH_LT_OK(hipblasLtMatmulDescSetAttribute(L->desc, HIPBLASLT_MATMUL_DESC_TRANSB, &tb, sizeof tb)))
+    return cleanup();
+  hipblasLtMatmulPreference_t pref = nullptr;
+  if (!MH_LT_OK(hipblasLtMatmulPreferenceCreate(&pref))) return cleanup();
+  const uint64_t wsz = kLtWorkspace;
+  (void)hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof wsz);
+  const int32_t bc = (int32_t)d.B;
+  const int64_t sa = ga.a_b_stride, sb = ga.b_b_stride, sc = p.Mp * p.Np;
+  const int64_t ld = ga.rowbytes;  // int8: bytes == elements
+  bool ok = true;
+  L->g.resize((size_t)p.slices);
+  for (int dd = 2; dd <= p.slices + 1 && ok; ++dd) {
+    LtGroups::Group& G = L->g[(size_t)(dd - 2)];
+    const uint64_t K = (uint64_t)(dd - 1) * (uint64_t)p.Kp;
+    // C^T (N x M, column-major, ld Np) = op(B window)^T (N x K) * A window (K x M)
+    ok = MH_LT_OK(hipblasLtMatrixLayoutCreate(&G.la, HIP_R_8I, K, (uint64_t)d.N, ld)) &&
+         MH_LT_OK(hipblasLtMatrixLayoutCreate(&G.lb, HIP_R_8I, K, (uint64_t)d.M, ld)) &&
+         MH_LT_OK(hipblasLtMatrixLayoutCreate(&G.lc, HIP_R_32I, (uint64_t)d.N, (uint64_t)d.M, p.Np));
+    if (ok && bc > 1) {
+      ok = MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.la, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc)) &&
+           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.la, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &sb,
+                                                      sizeof sb)) &&
+           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lb, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc)) &&
+           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lb, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &sa,
+                                                      sizeof sa)) &&
+           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lc, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc)) &&
+           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lc, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &sc,
+                                                      sizeof sc));
+    }
+    if (!ok) break;
+    hipblasLtMatmulHeuristicResult_t res[kLtCandidates];
+    int n = 0;
+    ok = MH_LT_OK(hipblasLtMatmulAlgoGetHeuristic((hipblasLtHandle_t)h.lt, L->desc, G.la, G.lb, G.lc, G.lc, pref,
+                                                  kLtCandidates, res, &n)) &&
+         n > 0;
+    for (int i = 0; i < n && ok; ++i) G.cand[i] = res[i].algo;
+    G.ncand = n;
+  }
+  (void)hipblasLtMatmulPreferenceDestroy(pref);
+  if (!ok) return cleanup();
+  p.lt = L.release();
+  return true;
+}
+
+}  // namespace
+
+// Runs the groups as library GEMMs; false leaves them to the own kernel.  The
+// first run of a plan times every candidate algorithm of each group on the
+// plan's own packed operands (synchronizing once) and keeps the fastest.
+bool run_lib_groups(Handle& h, Plan& p, const SlicedArgs& ga) {
+  if (!p.lt && !setup_lt(h, p, ga)) {
+    p.split_lib = false;
+    return false;
+  }
+  hipblasLtHandle_t lt = (hipblasLtHandle_t)h.lt;
+  const int32_t one = 1, zero = 0;
+  const int S = p.slices;
+  const int64_t gstride = (int64_t)p.d.B * p.Mp * p.Np * 4;
+  auto launch = [&](int gi, const hipblasLtMatmulAlgo_t* algo) {
+    const int ap = p.lib_groups[3 * gi], bp = p.lib_groups[3 * gi + 1], dd = p.lib_groups[3 * gi + 2];
+    LtGroups::Group& G = p.lt->g[(size_t)(dd - 2)];
+    const uint8_t* Aw = ga.QA + ap * ga.a_plane_stride;                                      // slices 1..d-1
+    const uint8_t* Bw = ga.QB + bp * ga.b_plane_stride + (int64_t)(S - dd + 1) * ga.slot_bytes;  // d-1..1
+    char* Cg = (char*)ga.out + gi * gstride;
+    return hipblasLtMatmul(lt, p.lt->desc, &one, Bw, G.la, Aw, G.lb, &zero, Cg, G.lc, Cg, G.lc, algo, h.lt_ws,
+                           kLtWorkspace, h.stream);
+  };
+  bool tuned = true;
+  for (auto& G : p.lt->g) tuned = tuned && (G.best >= 0 || G.ncand == 0);
+  if (!tuned) {
+    hipEvent_t e0, e1;
+    MH_HIP_CHECK(hipEventCreate(&e0));
+    MH_HIP_CHECK(hipEventCreate(&e1));
+    for (size_t gi = 0; gi < p.lib_groups.size() / 3; ++gi) {
+      LtGroups::Group& G = p.lt->g[(size_t)(p.lib_groups[3 * gi + 2] - 2)];
+      if (G.best >= 0) continue;
+      float best_ms = 1e30f;
+      for (int c = 0; c < G.ncand; ++c) {
+        if (!MH_LT_OK(launch((int)gi, &G.cand[c]))) continue;  // warm-up; an algorithm that fails is skipped
+        MH_HIP_CHECK(hipEventRecord(e0, h.stream));
+        bool good = true;
+        for (int r = 0; r < 2 && good; ++r) good = MH_LT_OK(launch((int)gi, &G.cand[c]));
+        MH_HIP_CHECK(hipEventRecord(e1, h.stream));
+        MH_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        MH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (good && ms < best_ms) {
+          best_ms = ms;
+          G.best = c;
+        }
+      }
+      if (G.best < 0) {  // no candidate ran: the own kernel takes the plan
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        release_split_lib(p);
+        p.split_lib = false;
+        return false;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  for (size_t gi = 0; gi < p.lib_groups.size() / 3; ++gi) {
+    LtGroups::Group& G = p.lt->g[(size_t)(p.lib_groups[3 * gi + 2] - 2)];
+    if (!MH_LT_OK(launch((int)gi, &G.cand[G.best])))
+      fail(MUSCLE_HIP_HIP_ERROR, "hipBLASLt int8 GEMM failed");
+  }
+  return true;
+}
+
 void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca, bool cb, void* ws) {
   const ContractionDesc& d = p.d;
   const WsLayout w = ws_layout(p);
@@ -529,9 +737,9 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
     hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((cols + 3) / 4)), dim3(NT), 0, s, sb);
   }
   auto launch_pack = [&](const void* X, int rowt, int kt, int bt, int32_t* exps, void* Q, int64_t R, int64_t Rp,
-                         bool cplx, bool r_major, int planes) {
+                         bool cplx, bool r_major, int planes, bool rev) {
     PackArgs pa{X, p.tab(rowt), p.tab(kt), p.tab(bt), scaled ? exps : nullptr, Q, (int32_t)R, (int32_t)d.K,
-                (int32_t)d.B, (int32_t)Rp, (int32_t)p.Kp, cplx, r_major, planes, p.slices, p.split_mode};
+                (int32_t)d.B, (int32_t)Rp, (int32_t)p.Kp, cplx, r_major, planes, p.slices, p.split_mode, rev};
     dim3 grid((unsigned)(p.Kp / PK), (unsigned)(Rp / PR), (unsigned)d.B);
     switch (p.split_mode) {
       case MODE_I8: hipLaunchKernelGGL(pack_kernel<MODE_I8>, grid, dim3(NT), 0, s, pa); break;
@@ -540,19 +748,21 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
     }
   };
   // A rows are M, its "row-major" gather direction is M when A is not K-major
-  launch_pack(A, TAB_ROWA, TAB_KA, TAB_BA, expA, base + w.QA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a);
-  launch_pack(B, TAB_COLB, TAB_KB, TAB_BB, expB, base + w.QB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b);
+  launch_pack(A, TAB_ROWA, TAB_KA, TAB_BA, expA, base + w.QA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a, false);
+  launch_pack(B, TAB_COLB, TAB_KB, TAB_BB, expB, base + w.QB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b, true);
 
-  const int64_t rowbytes = p.Kp * p.esize;
+  const int64_t slot = p.Kp * p.esize, rowbytes = slot * p.slices;
   SlicedArgs ga;
   std::memset(&ga, 0, sizeof ga);
   ga.QA = (const uint8_t*)(base + w.QA);
   ga.QB = (const uint8_t*)(base + w.QB);
   ga.a_b_stride = p.Mp * rowbytes;
   ga.b_b_stride = p.Np * rowbytes;
-  ga.a_ps_stride = ga.a_b_stride * d.B;
-  ga.b_ps_stride = ga.b_b_stride * d.B;
-  ga.rowbytes = (int32_t)rowbytes;
+  ga.a_plane_stride = ga.a_b_stride * d.B;
+  ga.b_plane_stride = ga.b_b_stride * d.B;
+  ga.rowbytes = rowbytes;
+  ga.slot_bytes = (int32_t)slot;
+  ga.S = p.slices;
   ga.segs = (const SliceSeg*)p.split_dev;
   ga.groups = (const SliceGroup*)((char*)p.split_dev + p.dev_groups_off);
   ga.out = base + w.G32;
@@ -562,11 +772,13 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
   ga.tiles_m = (int32_t)(p.Mp / SBM);
   ga.tiles_n = (int32_t)(p.Np / SBN);
   ga.esize = p.esize;
-  dim3 grid((unsigned)(ga.tiles_m * ga.tiles_n), (unsigned)d.B, (unsigned)p.ngroups);
-  switch (p.split_mode) {
-    case MODE_I8: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_I8>, grid, dim3(NT), 0, s, ga); break;
-    case MODE_BF16: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_BF16>, grid, dim3(NT), 0, s, ga); break;
-    default: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_F16>, grid, dim3(NT), 0, s, ga); break;
+  if (!(p.split_lib && run_lib_groups(h, p, ga))) {
+    dim3 grid((unsigned)(ga.tiles_m * ga.tiles_n), (unsigned)d.B, (unsigned)p.ngroups);
+    switch (p.split_mode) {
+      case MODE_I8: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_I8>, grid, dim3(NT), 0, s, ga); break;
+      case MODE_BF16: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_BF16>, grid, dim3(NT), 0, s, ga); break;
+      default: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_F16>, grid, dim3(NT), 0, s, ga); break;
+    }
   }
   CombineArgs cm{base + w.G32,
                  (const double*)((char*)p.split_dev + p.dev_coef_off),

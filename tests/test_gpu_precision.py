@@ -25,6 +25,8 @@ SHAPES = [
     ("hyper", ["i", "j", "h"], (40, 36, 3), ["j", "k", "h"], (36, 50, 3), ["h", "k", "i"]),
     ("interleaved d=8", ["a1", "c1", "a3", "c2", "a5", "c3"], (8,) * 6, ["c1", "b1", "c2", "b2", "c3", "b3"], (8,) * 6,
      ["a1", "b1", "a3", "b2", "a5", "b3"]),
+    # M and N above 512 and not multiples of the 256-wide sliced tile
+    ("ragged wide", ["i", "j"], (600, 130), ["j", "k"], (130, 530), ["k", "i"]),
 ]
 
 
@@ -59,6 +61,28 @@ def test_ozaki_int8(shape, T, target, S):
     C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=target)
     assert info["kernel"] == 3 and info["slices"] == S
     assert rel_err(C, ref) <= target, rel_err(C, ref)
+
+
+# large enough (M*N*K*batch >= 1e9) for the slice products to run as library
+# GEMMs; ragged extents, a batch (hyper) mode and a second call on the tuned plan
+LIB_SHAPES = [
+    ("square 1024", ["i", "j"], (1024, 1024), ["j", "k"], (1024, 1024), ["i", "k"]),
+    ("ragged", ["i", "j"], (1000, 1000), ["j", "k"], (1000, 1100), ["k", "i"]),
+    ("hyper", ["i", "j", "h"], (1024, 512, 2), ["j", "k", "h"], (512, 1024, 2), ["h", "k", "i"]),
+]
+
+
+@pytest.mark.parametrize("target, S", [(1e-10, 5), (1e-12, 6)])
+@pytest.mark.parametrize("shape", LIB_SHAPES, ids=lambda s: s[0])
+def test_ozaki_int8_library_groups(shape, target, S):
+    name, ia, sa, ib, sb, ic = shape
+    rng = np.random.default_rng(8)
+    A, B = randn(rng, sa, np.complex128), randn(rng, sb, np.complex128)
+    ref = bb.binary_einsum_general(ic, A, ia, B, ib)
+    for call in range(2):  # the first call tunes the library algorithms
+        C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=target)
+        assert info["kernel"] == 5 and info["slices"] == S, info
+        assert rel_err(C, ref) <= target, (call, rel_err(C, ref))
 
 
 @pytest.mark.parametrize("ta, tb", [(np.float64, np.complex128), (np.complex128, np.float64)])
