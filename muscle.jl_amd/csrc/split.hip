@@ -56,6 +56,10 @@ struct ScaleArgs {
 };
 
 // One wavefront per (batch, row): e = floor(log2(max_k |re| + |im|)) + bias.
+// (For row-contiguous operands neighbouring wavefronts read neighbouring
+// elements of the same cache lines, which L2 serves.)
+__device__ inline int32_t scale_exp(double mx, int bias) { return mx > 0.0 ? ilogb(mx) + bias : 0; }
+
 __global__ __launch_bounds__(NT) void scale_kernel(ScaleArgs p) {
   const int64_t row = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(NT) void scale_kernel(ScaleArgs p) {
     mx = fmax(mx, fabs(v.re) + fabs(v.im));
   }
   for (int s = 32; s > 0; s >>= 1) mx = fmax(mx, __shfl_xor(mx, s, 64));
-  if (lane == 0) p.exps[row] = mx > 0.0 ? ilogb(mx) + p.bias : 0;
+  if (lane == 0) p.exps[row] = scale_exp(mx, p.bias);
 }
 
 // ------------------------------------------------------------------ pack
@@ -85,58 +89,103 @@ struct PackArgs {
 
 constexpr int PR = 32, PK = 64;  // pack tile: 32 rows x 64 k
 
-__device__ inline int digit255(double v) {
-  // round to nearest, ties toward zero, clamped to the int8 range
-  double q = rint(v);
-  if (fabs(v - trunc(v)) == 0.5) q = trunc(v);
-  return (int)fmin(127.0, fmax(-127.0, q));
+// Slices of one value into nps bytes / halfwords (per plane, S slices).
+template <int MODE>
+__device__ inline void slice_value(double res, int s_count, uint32_t* out, int shift) {
+  // 1.5 * 2^52: adding it rounds |t| <= 127 to an integer (ties to even) that
+  // sits in the low bits of the sum's representation
+  constexpr double kMagic = 6755399441055744.0;
+  for (int s = 0; s < s_count; ++s) {
+    if constexpr (MODE == MODE_I8) {
+      // digit: t rounded to nearest within [-127, 127]; res = t - q keeps
+      // |res| <= 0.5 (a clamped +-127.5 leaves +-0.5 for the next digit)
+      const double t = res * 255.0;
+      const double m = fmin(fmax(t, -127.0), 127.0) + kMagic;
+      const int32_t q = (int32_t)(uint32_t)__double_as_longlong(m);
+      res = t - (m - kMagic);
+      out[s] |= (uint32_t)(uint8_t)(int8_t)q << shift;
+    } else if constexpr (MODE == MODE_BF16) {
+      const __bf16 q = (__bf16)(float)res;
+      res -= (double)(float)q;
+      out[s] |= (uint32_t)__builtin_bit_cast(uint16_t, q) << shift;
+    } else {
+      const _Float16 q = (_Float16)(float)res;
+      res -= (double)(float)q;
+      out[s] |= (uint32_t)__builtin_bit_cast(uint16_t, q) << shift;
+    }
+  }
 }
 
+// One 32-row x 64-k tile: every thread gathers 2 x 4 consecutive-k values of
+// one row (rows side by side for row-contiguous operands, k-quads side by
+// side otherwise), slices them, and stores each plane-slice's 4 values as one
+// (int8) or two (bf16/f16) 32-bit LDS words; rows are PK*ES/4 + 1 words apart,
+// which keeps those stores free of bank conflicts.  The tile then leaves as
+// 16-byte chunks of K-contiguous packed rows.
 template <int MODE>
 __global__ __launch_bounds__(NT) void pack_kernel(PackArgs p) {
   constexpr int ES = MODE == MODE_I8 ? 1 : 2;
   constexpr int MAXPS = MODE == MODE_I8 ? 3 * 7 : (MODE == MODE_BF16 ? 3 * 3 : 3 * 2);
-  __shared__ __attribute__((aligned(16))) uint8_t tile[MAXPS][PR][PK * ES];
+  constexpr int WPQ = ES;                 // LDS words per k-quad
+  constexpr int RW = PK * ES / 4 + 1;     // LDS words per tile row
+  constexpr int MAXS = MODE == MODE_I8 ? 7 : 3;
+  __shared__ uint32_t tile[MAXPS][PR][RW];
   const int k0 = blockIdx.x * PK, r0 = blockIdx.y * PR, b = blockIdx.z;
   const int nps = p.planes * p.S;
-  for (int i = 0; i < (PR * PK) / NT; ++i) {
-    const int e = threadIdx.x + NT * i;
-    int rl, kl;
+  for (int h = 0; h < 2; ++h) {
+    int rl, kq;
     if (p.r_major) {
-      rl = e % PR;
-      kl = e / PR;
+      rl = threadIdx.x % PR;
+      kq = threadIdx.x / PR + 8 * h;
     } else {
-      kl = e % PK;
-      rl = e / PK;
+      kq = threadIdx.x % (PK / 4);
+      rl = threadIdx.x / (PK / 4) + 16 * h;
     }
-    const int r = r0 + rl, k = k0 + kl;
-    double v[3] = {0.0, 0.0, 0.0};
-    if (r < p.R && k < p.K) {
-      const int64_t off = p.bX[b] + p.rowX[r] + p.kX[k];
-      const Cplx<double> x = p.cplx ? load_elem<double, true>(p.X, off) : load_elem<double, false>(p.X, off);
-      const double sc = p.exps ? ldexp(1.0, -p.exps[(int64_t)b * p.R + r]) : 1.0;
-      v[0] = x.re * sc;
-      v[1] = x.im * sc;
-      v[2] = (x.re + x.im) * sc;
-    }
-    for (int pl = 0; pl < p.planes; ++pl) {
-      double res = v[pl];
-      for (int s = 0; s < p.S; ++s) {
-        uint8_t* dst = &tile[pl * p.S + s][rl][kl * ES];
-        if constexpr (MODE == MODE_I8) {
-          const double t = res * 255.0;
-          const int q = digit255(t);
-          res = t - (double)q;
-          *reinterpret_cast<int8_t*>(dst) = (int8_t)q;
-        } else if constexpr (MODE == MODE_BF16) {
-          const __bf16 q = (__bf16)(float)res;
-          res -= (double)(float)q;
-          *reinterpret_cast<__bf16*>(dst) = q;
-        } else {
-          const _Float16 q = (_Float16)(float)res;
-          res -= (double)(float)q;
-          *reinterpret_cast<_Float16*>(dst) = q;
+    const int r = r0 + rl;
+    uint32_t acc[3][MAXS][WPQ];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+#pragma unroll
+        for (int w = 0; w < WPQ; ++w) acc[pl][s][w] = 0;
+    const double sc = (r < p.R && p.exps) ? ldexp(1.0, -p.exps[(int64_t)b * p.R + r]) : 1.0;
+    const int64_t rbase = r < p.R ? p.bX[b] + p.rowX[r] : 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + 4 * kq + e;
+      double v[3] = {0.0, 0.0, 0.0};
+      if (r < p.R && k < p.K) {
+        const int64_t off = rbase + p.kX[k];
+        const Cplx<double> x = p.cplx ? load_elem<double, true>(p.X, off) : load_elem<double, false>(p.X, off);
+        v[0] = x.re * sc;
+        v[1] = x.im * sc;
+        v[2] = (x.re + x.im) * sc;
+      }
+      const int word = (e * ES) / 4, shift = ((e * ES) % 4) * 8;
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) {
+        if (pl >= p.planes) break;
+        uint32_t tmp[MAXS];
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s) tmp[s] = 0;
+        slice_value<MODE>(v[pl], p.S, tmp, shift);
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s) {
+#pragma unroll
+          for (int w = 0; w < WPQ; ++w)
+            if (w == word) acc[pl][s][w] |= tmp[s];
         }
+      }
+    }
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+      if (pl >= p.planes) break;
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s) {
+        if (s >= p.S) break;
+#pragma unroll
+        for (int w = 0; w < WPQ; ++w) tile[pl * p.S + s][rl][kq * WPQ + w] = acc[pl][s][w];
       }
     }
   }
@@ -151,7 +200,8 @@ __global__ __launch_bounds__(NT) void pack_kernel(PackArgs p) {
     const int ps = c / (CPR * PR);
     const int pl = ps / p.S, sl = ps - pl * p.S;
     const int pos = p.rev ? p.S - 1 - sl : sl;
-    const uint4 val = *reinterpret_cast<const uint4*>(&tile[ps][rl][ch * 16]);
+    const uint32_t* src = &tile[ps][rl][ch * 4];
+    const uint4 val = make_uint4(src[0], src[1], src[2], src[3]);
     uint8_t* out = (uint8_t*)p.Q + (((int64_t)pl * p.B + b) * p.Rp + r0 + rl) * rowbytes + pos * slot +
                    (int64_t)k0 * ES + ch * 16;
     *reinterpret_cast<uint4*>(out) = val;
@@ -346,16 +396,26 @@ struct CombineArgs {
   void* C;
   OffTab rowC, colC, bC;
   int32_t G, B, Mp, Np, M, N, is_int, c_complex;
+  FastDiv divN, divM;  // used when B * M * N < 2^32
+  int32_t small;
 };
 
 __global__ __launch_bounds__(NT) void combine_kernel(CombineArgs p) {
   const int64_t total = (int64_t)p.B * p.M * p.N;
   const int64_t o = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (o >= total) return;
-  const int64_t t = o / p.N;
-  const int n = (int)(o - t * p.N);
-  const int b = (int)(t / p.M);
-  const int m = (int)(t - (int64_t)b * p.M);
+  int n, m, b;
+  if (p.small) {
+    const uint32_t t = p.divN.div((uint32_t)o);
+    n = (int)((uint32_t)o - t * (uint32_t)p.N);
+    b = (int)p.divM.div(t);
+    m = (int)(t - (uint32_t)b * (uint32_t)p.M);
+  } else {
+    const int64_t t = o / p.N;
+    n = (int)(o - t * p.N);
+    b = (int)(t / p.M);
+    m = (int)(t - (int64_t)b * p.M);
+  }
   const int64_t plane = (int64_t)p.B * p.Mp * p.Np;
   const int64_t at = ((int64_t)b * p.Mp + m) * p.Np + n;
   double re = 0.0, im = 0.0;
@@ -549,7 +609,7 @@ void upload_split_tables(Plan& p, hipStream_t s) {
 // holding slices 1..d-1 times the K window of B holding slices d-1..1.
 
 constexpr size_t kLtWorkspace = size_t(32) << 20;
-constexpr int kLtCandidates = 8;
+constexpr int kLtCandidates = 16;
 
 struct LtGroups {
   hipblasLtMatmulDesc_t desc = nullptr;
@@ -559,6 +619,11 @@ struct LtGroups {
     int ncand = 0, best = -1;
   };
   std::vector<Group> g;  // index d - 2
+  // With no hyper (batch) modes, the complex products of one weight d share
+  // a shape and their planes are evenly spaced: one call per d, batched over
+  // the products (nprod > 1), instead of one call per product.
+  int nprod = 1;
+  bool plane_batched = false;
 };
 
 void release_split_lib(Plan& p) {
@@ -613,8 +678,24 @@ bool setup_lt(Handle& h, Plan& p, const SlicedArgs& ga) {
   if (!MH_LT_OK(hipblasLtMatmulPreferenceCreate(&pref))) return cleanup();
   const uint64_t wsz = kLtWorkspace;
   (void)hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof wsz);
-  const int32_t bc = (int32_t)d.B;
-  const int64_t sa = ga.a_b_stride, sb = ga.b_b_stride, sc = p.Mp * p.Np;
+  // products: (a plane, b plane) of group entries with weight 2, in order
+  const int ng = (int)(p.lib_groups.size() / 3);
+  std::vector<int> pa, pb;
+  for (int gi = 0; gi < ng; ++gi)
+    if (p.lib_groups[3 * gi + 2] == 2) {
+      pa.push_back(p.lib_groups[3 * gi]);
+      pb.push_back(p.lib_groups[3 * gi + 1]);
+    }
+  L->nprod = (int)pa.size();
+  bool linear = L->nprod > 1;
+  for (int i = 2; i < L->nprod; ++i)
+    linear = linear && pa[i] - pa[i - 1] == pa[1] - pa[0] && pb[i] - pb[i - 1] == pb[1] - pb[0];
+  L->plane_batched = d.B == 1 && linear;
+  const int32_t bc = L->plane_batched ? (int32_t)L->nprod : (int32_t)d.B;
+  const int64_t groups_per_prod = ng / std::max(1, L->nprod);
+  const int64_t sa = L->plane_batched ? (pa.size() > 1 ? (pa[1] - pa[0]) * ga.a_plane_stride : 0) : ga.a_b_stride;
+  const int64_t sb = L->plane_batched ? (pb.size() > 1 ? (pb[1] - pb[0]) * ga.b_plane_stride : 0) : ga.b_b_stride;
+  const int64_t sc = L->plane_batched ? groups_per_prod * p.Mp * p.Np : p.Mp * p.Np;  // int32 elements
   const int64_t ld = ga.rowbytes;  // int8: bytes == elements
   bool ok = true;
   L->g.resize((size_t)p.slices);
@@ -665,6 +746,9 @@ bool run_lib_groups(Handle& h, Plan& p, const SlicedArgs& ga) {
   const int32_t one = 1, zero = 0;
   const int S = p.slices;
   const int64_t gstride = (int64_t)p.d.B * p.Mp * p.Np * 4;
+  // in plane-batched mode only the first product's entries are launched
+  const int ng = (int)(p.lib_groups.size() / 3);
+  const int ncall = p.lt->plane_batched ? ng / p.lt->nprod : ng;
   auto launch = [&](int gi, const hipblasLtMatmulAlgo_t* algo) {
     const int ap = p.lib_groups[3 * gi], bp = p.lib_groups[3 * gi + 1], dd = p.lib_groups[3 * gi + 2];
     LtGroups::Group& G = p.lt->g[(size_t)(dd - 2)];
@@ -680,7 +764,7 @@ bool run_lib_groups(Handle& h, Plan& p, const SlicedArgs& ga) {
     hipEvent_t e0, e1;
     MH_HIP_CHECK(hipEventCreate(&e0));
     MH_HIP_CHECK(hipEventCreate(&e1));
-    for (size_t gi = 0; gi < p.lib_groups.size() / 3; ++gi) {
+    for (int gi = 0; gi < ncall; ++gi) {
       LtGroups::Group& G = p.lt->g[(size_t)(p.lib_groups[3 * gi + 2] - 2)];
       if (G.best >= 0) continue;
       float best_ms = 1e30f;
@@ -709,9 +793,9 @@ bool run_lib_groups(Handle& h, Plan& p, const SlicedArgs& ga) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  for (size_t gi = 0; gi < p.lib_groups.size() / 3; ++gi) {
+  for (int gi = 0; gi < ncall; ++gi) {
     LtGroups::Group& G = p.lt->g[(size_t)(p.lib_groups[3 * gi + 2] - 2)];
-    if (!MH_LT_OK(launch((int)gi, &G.cand[G.best])))
+    if (!MH_LT_OK(launch(gi, &G.cand[G.best])))
       fail(MUSCLE_HIP_HIP_ERROR, "hipBLASLt int8 GEMM failed");
   }
   return true;
@@ -727,14 +811,14 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
   const int bias = p.split_mode == MODE_I8 ? 2 : 1;
   hipStream_t s = h.stream;
   if (scaled) {
-    ScaleArgs sa{A, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), expA, (int32_t)d.M, (int32_t)d.K, (int32_t)d.B, ca,
-                 bias};
-    const int64_t rows = d.M * d.B;
-    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(NT), 0, s, sa);
-    ScaleArgs sb{B, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), expB, (int32_t)d.N, (int32_t)d.K, (int32_t)d.B, cb,
-                 bias};
-    const int64_t cols = d.N * d.B;
-    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((cols + 3) / 4)), dim3(NT), 0, s, sb);
+    auto launch_scale = [&](const ScaleArgs& sa) {
+      const int64_t rows = (int64_t)sa.R * sa.B;
+      hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(NT), 0, s, sa);
+    };
+    launch_scale(ScaleArgs{A, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), expA, (int32_t)d.M, (int32_t)d.K,
+                           (int32_t)d.B, ca, bias});
+    launch_scale(ScaleArgs{B, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), expB, (int32_t)d.N, (int32_t)d.K,
+                           (int32_t)d.B, cb, bias});
   }
   auto launch_pack = [&](const void* X, int rowt, int kt, int bt, int32_t* exps, void* Q, int64_t R, int64_t Rp,
                          bool cplx, bool r_major, int planes, bool rev) {
@@ -795,7 +879,10 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
                  (int32_t)d.M,
                  (int32_t)d.N,
                  p.split_mode == MODE_I8,
-                 ca || cb};
+                 ca || cb,
+                 FastDiv((uint32_t)std::max<int64_t>(1, std::min<int64_t>(d.N, UINT32_MAX))),
+                 FastDiv((uint32_t)std::max<int64_t>(1, std::min<int64_t>(d.M, UINT32_MAX))),
+                 d.B * d.M * d.N < (int64_t(1) << 32)};
   const int64_t total = d.B * d.M * d.N;
   hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0, s, cm);
 }
