@@ -369,7 +369,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only", default="all",
-                    help="comma list of: ladder_c2,c2_c32,c2h_c64,c1_c64,ladder_c1,batched,served "
+                    help="comma list of: ladder_c2,c2_c32,c2h_c64,dims2,c1_c64,ladder_c1,batched,served "
                          "(the headline always runs)")
     ap.add_argument("--precision", default="native", help="arithmetic of the headline workload")
     ap.add_argument("--target", type=float, default=1e-10)
@@ -470,6 +470,28 @@ def main() -> int:
         extra["c2h_c64"] = {"ms": round(m2, 4), "tflops": round(inf2["flops"] / (m2 * 1e-3) / 1e12, 3),
                             "batch": inf2["batch"], "m": inf2["m"], "n": inf2["n"], "k": inf2["k"]}
         del c2
+    if want("dims2"):
+        # circuit-like contraction, every mode of extent 2 in random orders:
+        # M = N = K = 2^12 (rank-24 operands), the packed-operand GEMM path
+        rng = np.random.default_rng(3)
+        labs = [f"x{i}" for i in range(36)]
+        Lm, Rn, Ck = labs[:12], labs[12:24], labs[24:]
+        ia = list(rng.permutation(Lm + Ck))
+        ib = list(rng.permutation(Ck + Rn))
+        ic = list(rng.permutation(Lm + Rn))
+        res = {}
+        for nm, T in (("c64", np.complex128), ("c32", np.complex64)):
+            Ad = (rng.standard_normal(2 ** 24) + 1j * rng.standard_normal(2 ** 24)).astype(T) / 64
+            Bd = (rng.standard_normal(2 ** 24) + 1j * rng.standard_normal(2 ** 24)).astype(T) / 64
+            ad = put(Ad.reshape((2,) * 24, order="F"), ia, dev)
+            bd = put(Bd.reshape((2,) * 24, order="F"), ib, dev)
+            m2, c2, inf2 = contraction_case(ad, bd, {"out": ic}, plain, max(3, args.steps // 2), 2)
+            peak = PEAK_TFLOPS["fp64" if nm == "c64" else "fp32"]
+            tf = inf2["flops"] / (m2 * 1e-3) / 1e12
+            res[nm] = {"ms": round(m2, 4), "tflops": round(tf, 3), "frac_peak": round(tf / peak, 4)}
+            del ad, bd, c2
+        extra["dims2_4096cube"] = res
+        log(f"dims2: {res}")
     if want("c1_c64"):
         A1, ia1, B1, ib1, dims, ic1 = wl.config1_inputs(0)
         a1, b1 = put(A1, ia1, dev), put(B1, ib1, dev)

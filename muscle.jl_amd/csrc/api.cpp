@@ -164,7 +164,11 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   if (precision != MUSCLE_HIP_PRECISION_NATIVE && d.has_b && is_double(d.dt[2]) && d.M >= 16 && d.N >= 16 &&
       d.K >= 16 && fits32 && d.B <= 65535) {
     p->kind = Plan::SPLIT;
-  } else if (d.has_b && real_fp && fits32 && d.M >= 32 && d.N >= 32 && d.K >= 8) {
+  } else if (d.has_b && real_fp && fits32 &&
+             ((d.M >= 32 && d.N >= 32 && d.K >= 8) || (d.K >= 16 && d.M * d.N >= 4096))) {
+    // skinny products (one side under 32, e.g. 2^18 x 8 x 2^10) also go to the
+    // MFMA tiles: they read each operand once, where the reduction kernel
+    // would re-read the long side once per output of the short one
     p->kind = Plan::GEMM;
   } else {
     p->kind = Plan::REDUCE;
@@ -202,6 +206,67 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   p->kchunk = (d.K + nsplit - 1) / nsplit;
   if (p->kchunk == 0) p->kchunk = 1;
   return p;
+}
+
+// Decides whether a GEMM plan packs its operands first, and builds the plan
+// of the contraction on the packed copies.  An operand is packed when the
+// contraction is large (2^27 index points or more), its gather direction
+// runs contiguous for less than 64 bytes, every one of its dims has a
+// stride, and the copy is at most 2 GiB.
+void consider_packing(Plan& p) {
+  if (p.kind != Plan::GEMM || p.fp32_math || p.precision != MUSCLE_HIP_PRECISION_NATIVE) return;
+  const ContractionDesc& d = p.d;
+  if ((double)d.M * (double)d.N * (double)d.K * (double)d.B < (double)(int64_t(1) << 27) || d.B > 65535) return;
+  auto lowest = [](const std::vector<Dim>& v, int w) {
+    int64_t b = INT64_MAX;
+    for (auto& x : v) b = std::min(b, x.s[w] < 0 ? -x.s[w] : x.s[w]);
+    return b;
+  };
+  auto run = [](const std::vector<Dim>& v, int w) {
+    return (!v.empty() && (v[0].s[w] == 1 || v[0].s[w] == -1)) ? v[0].extent : int64_t(1);
+  };
+  auto all_strided = [](const std::vector<Dim>& v, int w) {
+    for (auto& x : v)
+      if (x.s[w] == 0) return false;
+    return true;
+  };
+  const size_t es[2] = {dtype_size(d.dt[0]), dtype_size(d.dt[1])};
+  const std::vector<Dim>* rows[2] = {&d.m, &d.n};
+  const bool kmaj[2] = {p.a_kmajor, p.b_kmajor};
+  bool pack[2] = {false, false}, kfast[2] = {false, false};
+  for (int w = 0; w < 2; ++w) {
+    const int64_t r = kmaj[w] ? run(d.k, w) : run(*rows[w], w);
+    const double bytes = (double)(w == 0 ? d.M : d.N) * (double)d.K * (double)d.B * (double)es[w];
+    pack[w] = (double)r * (double)es[w] < 64.0 && bytes <= (double)(int64_t(2) << 30) && all_strided(d.k, w) &&
+              all_strided(*rows[w], w) && all_strided(d.batch, w);
+    kfast[w] = lowest(d.k, w) < lowest(*rows[w], w);
+  }
+  if (!pack[0] && !pack[1]) return;
+  // strides of the packed copies: [batch][rows][K], K contiguous
+  ContractionDesc di = d;
+  for (int w = 0; w < 2; ++w) {
+    if (!pack[w]) continue;
+    int64_t cur = 1;
+    for (auto& x : di.k) {
+      x.s[w] = cur;
+      cur *= x.extent;
+    }
+    for (auto& x : (w == 0 ? di.m : di.n)) {
+      x.s[w] = cur;
+      cur *= x.extent;
+    }
+    for (auto& x : di.batch) {
+      x.s[w] = cur;
+      cur *= x.extent;
+    }
+  }
+  std::unique_ptr<Plan> in = build_contraction_plan(di, MUSCLE_HIP_PRECISION_NATIVE);
+  if (in->kind != Plan::GEMM) return;
+  p.pack_a = pack[0];
+  p.pack_b = pack[1];
+  p.pack_a_kfast = kfast[0];
+  p.pack_b_kfast = kfast[1];
+  p.inner = std::move(in);
 }
 
 namespace {
@@ -278,10 +343,18 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   if (d.swapped) std::swap(ka, kb);
   size_t need = cast_bytes(*ka, dbl) + (kb ? cast_bytes(*kb, dbl) : 0);
   const size_t csize = dtype_size(d.dt[2]);
+  // packed operand copies (element size after the precision promotion)
+  const size_t es_a = dtype_size(make_dtype(dbl, is_complex(ka->dtype)));
+  const size_t es_b = kb ? dtype_size(make_dtype(dbl, is_complex(kb->dtype))) : 0;
+  const size_t pack_a_off = need;
+  if (p.pack_a) need += align_up((size_t)(d.M * d.K * d.B) * es_a);
+  const size_t pack_b_off = need;
+  if (p.pack_b) need += align_up((size_t)(d.N * d.K * d.B) * es_b);
+  Plan& q = p.inner ? *p.inner : p;  // the plan whose tables the GEMM reads
   size_t partial_off = need;
   if (p.kind == Plan::REDUCE && p.nsplit > 1) need += align_up((size_t)(d.M * d.N * d.B) * csize * p.nsplit);
-  if (p.kind == Plan::GEMM && p.gemm_ksplit > 1)
-    need += align_up((size_t)(d.M * d.N * d.B) * csize * p.gemm_ksplit);
+  if (p.kind == Plan::GEMM && q.gemm_ksplit > 1)
+    need += align_up((size_t)(d.M * d.N * d.B) * csize * q.gemm_ksplit);
   size_t split_off = need;
   if (p.kind == Plan::SPLIT) need += split_workspace_bytes(p);
   if (need) h.ensure_workspace(need);
@@ -295,32 +368,48 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     return;
   }
   if (p.kind == Plan::GEMM) {
+    if (p.pack_a) {
+      char* buf = (char*)h.ws + pack_a_off;
+      launch_pack_operand((int)es_a, PackOpArgs{A, buf, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), (int32_t)d.M,
+                                                (int32_t)d.K, (int32_t)d.B, p.pack_a_kfast},
+                          h.stream);
+      check_launch();
+      A = buf;
+    }
+    if (p.pack_b) {
+      char* buf = (char*)h.ws + pack_b_off;
+      launch_pack_operand((int)es_b, PackOpArgs{B, buf, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), (int32_t)d.N,
+                                                (int32_t)d.K, (int32_t)d.B, p.pack_b_kfast},
+                          h.stream);
+      check_launch();
+      B = buf;
+    }
     GemmArgs g;
     std::memset(&g, 0, sizeof g);
     g.A = A;
     g.B = B;
     g.C = c.data;
-    g.rowA = p.tab(TAB_ROWA);
-    g.kA = p.tab(TAB_KA);
-    g.kB = p.tab(TAB_KB);
-    g.colB = p.tab(TAB_COLB);
-    g.rowC = p.tab(TAB_ROWC);
-    g.colC = p.tab(TAB_COLC);
+    g.rowA = q.tab(TAB_ROWA);
+    g.kA = q.tab(TAB_KA);
+    g.kB = q.tab(TAB_KB);
+    g.colB = q.tab(TAB_COLB);
+    g.rowC = q.tab(TAB_ROWC);
+    g.colC = q.tab(TAB_COLC);
     g.M = (int32_t)d.M;
     g.N = (int32_t)d.N;
     g.K = (int32_t)d.K;
-    g.a_kmajor = p.a_kmajor;
-    g.b_kmajor = p.b_kmajor;
+    g.a_kmajor = q.a_kmajor;
+    g.b_kmajor = q.b_kmajor;
     g.complex3m = h.complex_mult == 3;
-    g.ksplit = p.gemm_ksplit;
-    g.kchunk = p.gemm_kchunk;
+    g.ksplit = q.gemm_ksplit;
+    g.kchunk = q.gemm_kchunk;
     g.partial = (char*)h.ws + partial_off;
     // grid.y carries the batch; split launches beyond the 65535 limit
     for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
       g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);
-      g.bA = p.tab(TAB_BA, (uint32_t)b0);
-      g.bB = p.tab(TAB_BB, (uint32_t)b0);
-      g.bC = p.tab(TAB_BC, (uint32_t)b0);
+      g.bA = q.tab(TAB_BA, (uint32_t)b0);
+      g.bB = q.tab(TAB_BB, (uint32_t)b0);
+      g.bC = q.tab(TAB_BC, (uint32_t)b0);
       launch_gemm_mfma(dbl, p.fp32_math, is_complex(dta), is_complex(dtb), g, h.stream);
       check_launch();
     }
@@ -367,7 +456,9 @@ Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_h
   ContractionDesc d = kind == 1 ? lower_binary(*c, *a, *b) : lower_unary(*c, *a);
   std::unique_ptr<Plan> np = build_contraction_plan(d, precision);
   if (np->kind == Plan::SPLIT) prepare_split(*np, target);
+  consider_packing(*np);
   if (np->kind != Plan::EMPTY) upload_tables(*np, h.stream);
+  if (np->inner) upload_tables(*np->inner, h.stream);
   upload_split_tables(*np, h.stream);
   return h.insert_plan(key, std::move(np));
 }

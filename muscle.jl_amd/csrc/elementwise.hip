@@ -119,4 +119,52 @@ void launch_cast(int dt_src, int dt_dst, const void* src, void* dst, int64_t n, 
     hipLaunchKernelGGL((cast_kernel<double, float, true>), g, b, 0, s, src, dst, n);
 }
 
+// ------------------------------------------------------------------ operand pack
+// A 32-row x 64-k tile per workgroup.  The row and K classes list their
+// dims fastest-first, so the tile covers the operand's lowest-stride row
+// dims and lowest-stride K dims: it is read from one compact source region
+// (in the order of the smaller source stride) and written out as 32 rows of
+// 64 contiguous K elements.
+namespace {
+constexpr int PTR = 32, PTK = 64;
+
+template <typename E>
+__global__ __launch_bounds__(NT) void pack_operand_kernel(PackOpArgs p) {
+  __shared__ E tile[PTR][PTK + 1];
+  const int ktiles = (p.K + PTK - 1) / PTK;
+  const int k0 = (int)(blockIdx.x % ktiles) * PTK, r0 = (int)(blockIdx.x / ktiles) * PTR, b = blockIdx.z;
+  const E* src = reinterpret_cast<const E*>(p.src);
+  const int64_t base = p.bt[b];
+  for (int e = threadIdx.x; e < PTR * PTK; e += NT) {
+    int rl, kl;
+    if (p.k_fast) {
+      kl = e % PTK;
+      rl = e / PTK;
+    } else {
+      rl = e % PTR;
+      kl = e / PTR;
+    }
+    const int r = r0 + rl, k = k0 + kl;
+    if (r < p.R && k < p.K) tile[rl][kl] = src[base + p.row[r] + p.k[k]];
+  }
+  __syncthreads();
+  E* dst = reinterpret_cast<E*>(p.dst) + ((int64_t)b * p.R) * p.K;
+  for (int e = threadIdx.x; e < PTR * PTK; e += NT) {
+    const int kl = e % PTK, rl = e / PTK;
+    const int r = r0 + rl, k = k0 + kl;
+    if (r < p.R && k < p.K) dst[(int64_t)r * p.K + k] = tile[rl][kl];
+  }
+}
+}  // namespace
+
+void launch_pack_operand(int elem_bytes, const PackOpArgs& a, hipStream_t s) {
+  const int64_t tiles = ((int64_t)(a.K + PTK - 1) / PTK) * ((a.R + PTR - 1) / PTR);
+  const dim3 g((unsigned)tiles, 1, (unsigned)a.B), b(NT);
+  switch (elem_bytes) {
+    case 4: hipLaunchKernelGGL(pack_operand_kernel<uint32_t>, g, b, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(pack_operand_kernel<uint2>, g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL(pack_operand_kernel<uint4>, g, b, 0, s, a); break;
+  }
+}
+
 }  // namespace mh

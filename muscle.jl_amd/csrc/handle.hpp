@@ -45,6 +45,12 @@ struct Plan {
   // tile GEMM split-K
   int gemm_ksplit = 1;
   int gemm_kchunk = 0;
+  // Large GEMMs whose operands gather in short runs (e.g. many modes of
+  // extent 2 in random orders) first copy them K-contiguous; `inner` is the
+  // plan of the contraction on the packed copies (pack_a / pack_b say which).
+  bool pack_a = false, pack_b = false;
+  bool pack_a_kfast = false, pack_b_kfast = false;  // read order of the pack tiles
+  std::unique_ptr<Plan> inner;
   // reduction kernel configuration
   int gshift = 0;
   int nsplit = 1;

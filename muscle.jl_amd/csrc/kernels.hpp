@@ -108,4 +108,16 @@ void launch_hadamard(int dtA, int dtB, const EwArgs& e, hipStream_t s);
 // dst[i] = (dst type) src[i] for i in [0, n); complexness is preserved.
 void launch_cast(int dt_src, int dt_dst, const void* src, void* dst, int64_t n, hipStream_t s);
 
+// Copies a GEMM operand from its strided layout into [batch][rows][K] with K
+// contiguous (rows = M for A, N for B); source offsets come from the plan's
+// row, K and batch tables.  k_fast: read the tile with the K index fastest
+// (the source's smaller stride), else the row index fastest.
+struct PackOpArgs {
+  const void* src;
+  void* dst;
+  OffTab row, k, bt;
+  int32_t R, K, B, k_fast;
+};
+void launch_pack_operand(int elem_bytes, const PackOpArgs& a, hipStream_t s);
+
 }  // namespace mh

@@ -204,3 +204,63 @@ def test_config2_full_size(T):
     lhs = Cm.astype(np.complex128).sum(axis=1)
     rhs = Am @ Bm.sum(axis=1)
     assert rel_err(lhs, rhs) <= (1e-10 if T == np.complex128 else 1e-4)
+
+
+# ------------------------------------------------------------ packed operands
+def _dims2_case(seed, n_left, n_right, n_con, n_hyp=0):
+    """Circuit-like contraction: every mode of extent 2, random mode orders."""
+    rng = np.random.default_rng(seed)
+    L = [f"l{i}" for i in range(n_left)]
+    R = [f"r{i}" for i in range(n_right)]
+    C = [f"c{i}" for i in range(n_con)]
+    H = [f"h{i}" for i in range(n_hyp)]
+    ia = list(rng.permutation(L + C + H))
+    ib = list(rng.permutation(C + H + R))
+    ic = list(rng.permutation(L + R + H))
+    return ia, ib, ic
+
+
+# 2^27+ index points and 2-element contiguous runs: the operands are packed
+# K-contiguous before the GEMM
+@pytest.mark.parametrize("T", [np.complex128, np.complex64, np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(9, 9, 9, 0), (10, 8, 9, 1), (11, 9, 8, 0)], ids=str)
+def test_dims2_permuted_large(shape, T):
+    ia, ib, ic = _dims2_case(sum(shape) * 7 + 1, *shape)
+    rng = np.random.default_rng(21)
+    A, B = randn(rng, (2,) * len(ia), T), randn(rng, (2,) * len(ib), T)
+    ref = bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib)
+    c, got = _run(A, ia, B, ib, out=ic)
+    assert rel_err(got, ref) <= TOL[T], rel_err(got, ref)
+    c2, got2 = _run(A, ia, B, ib, out=ic)  # the cached plan
+    assert np.array_equal(got, got2)
+
+
+@pytest.mark.parametrize("ta, tb", [(np.float64, np.complex128), (np.complex64, np.float64)])
+def test_dims2_permuted_mixed_eltypes(ta, tb):
+    ia, ib, ic = _dims2_case(5, 9, 9, 9)
+    rng = np.random.default_rng(22)
+    A, B = randn(rng, (2,) * len(ia), ta), randn(rng, (2,) * len(ib), tb)
+    ref = bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib)
+    c, got = _run(A, ia, B, ib, out=ic)
+    assert rel_err(got, ref) <= 1e-12
+
+
+# skinny products: one open side under 32 (the MFMA tiles run partly empty)
+SKINNY = [
+    (["i", "j"], (8, 1024), ["j", "k"], (1024, 4096), ["i", "k"]),
+    (["i", "j"], (4096, 512), ["j", "k"], (512, 4), ["k", "i"]),
+    (["j"], (2048,), ["j", "k"], (2048, 8192), ["k"]),
+    (["i", "j", "h"], (16, 300, 3), ["j", "k", "h"], (300, 777, 3), ["h", "k", "i"]),
+]
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.float32])
+@pytest.mark.parametrize("case", SKINNY, ids=lambda c: "x".join(map(str, c[1] + c[3])))
+def test_skinny_products(case, T):
+    ia, sa, ib, sb, ic = case
+    rng = np.random.default_rng(23)
+    A, B = randn(rng, sa, T), randn(rng, sb, T)
+    ref = bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib)
+    c, got = _run(A, ia, B, ib, out=ic)
+    assert muscle.last_info()["kernel"] == 1
+    assert rel_err(got, ref) <= TOL[T], rel_err(got, ref)
