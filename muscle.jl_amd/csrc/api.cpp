@@ -211,7 +211,8 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
 // Decides whether a GEMM plan packs its operands first, and builds the plan
 // of the contraction on the packed copies.  An operand is packed when the
 // contraction is large (2^27 index points or more), its gather direction
-// runs contiguous for less than 64 bytes, every one of its dims has a
+// runs contiguous for less than 64 bytes, the tiles re-read it at least four
+// times (the other open side spans 256 or more), every one of its dims has a
 // stride, and the copy is at most 2 GiB.
 void consider_packing(Plan& p) {
   if (p.kind != Plan::GEMM || p.fp32_math || p.precision != MUSCLE_HIP_PRECISION_NATIVE) return;
@@ -237,8 +238,9 @@ void consider_packing(Plan& p) {
   for (int w = 0; w < 2; ++w) {
     const int64_t r = kmaj[w] ? run(d.k, w) : run(*rows[w], w);
     const double bytes = (double)(w == 0 ? d.M : d.N) * (double)d.K * (double)d.B * (double)es[w];
-    pack[w] = (double)r * (double)es[w] < 64.0 && bytes <= (double)(int64_t(2) << 30) && all_strided(d.k, w) &&
-              all_strided(*rows[w], w) && all_strided(d.batch, w);
+    const int64_t other = w == 0 ? d.N : d.M;
+    pack[w] = (double)r * (double)es[w] < 64.0 && other >= 256 && bytes <= (double)(int64_t(2) << 30) &&
+              all_strided(d.k, w) && all_strided(*rows[w], w) && all_strided(d.batch, w);
     kfast[w] = lowest(d.k, w) < lowest(*rows[w], w);
   }
   if (!pack[0] && !pack[1]) return;
