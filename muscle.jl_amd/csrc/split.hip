@@ -396,41 +396,66 @@ struct CombineArgs {
   void* C;
   OffTab rowC, colC, bC;
   int32_t G, B, Mp, Np, M, N, is_int, c_complex;
-  FastDiv divN, divM;  // used when B * M * N < 2^32
+  FastDiv divN, divM;  // by ceil(N / 4) and M, used when B * M * ceil(N / 4) < 2^32
   int32_t small;
 };
 
+// Every thread combines four consecutive columns n of one row (16-byte
+// loads from each group plane; the planes' rows are padded to 128 columns).
 __global__ __launch_bounds__(NT) void combine_kernel(CombineArgs p) {
-  const int64_t total = (int64_t)p.B * p.M * p.N;
+  const int Nq = (p.N + 3) >> 2;
+  const int64_t total = (int64_t)p.B * p.M * Nq;
   const int64_t o = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (o >= total) return;
-  int n, m, b;
+  int q, m, b;
   if (p.small) {
     const uint32_t t = p.divN.div((uint32_t)o);
-    n = (int)((uint32_t)o - t * (uint32_t)p.N);
+    q = (int)((uint32_t)o - t * (uint32_t)Nq);
     b = (int)p.divM.div(t);
     m = (int)(t - (uint32_t)b * (uint32_t)p.M);
   } else {
-    const int64_t t = o / p.N;
-    n = (int)(o - t * p.N);
+    const int64_t t = o / Nq;
+    q = (int)(o - t * Nq);
     b = (int)(t / p.M);
     m = (int)(t - (int64_t)b * p.M);
   }
+  const int n0 = 4 * q;
   const int64_t plane = (int64_t)p.B * p.Mp * p.Np;
-  const int64_t at = ((int64_t)b * p.Mp + m) * p.Np + n;
-  double re = 0.0, im = 0.0;
+  const int64_t at = ((int64_t)b * p.Mp + m) * p.Np + n0;
+  double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
   for (int g = 0; g < p.G; ++g) {
-    const double v = p.is_int ? (double)reinterpret_cast<const int32_t*>(p.G32)[g * plane + at]
-                              : (double)reinterpret_cast<const float*>(p.G32)[g * plane + at];
-    re = fma(p.coef[g], v, re);
-    im = fma(p.coef[p.G + g], v, im);
+    double v[4];
+    if (p.is_int) {
+      const int4 x = *reinterpret_cast<const int4*>(reinterpret_cast<const int32_t*>(p.G32) + g * plane + at);
+      v[0] = x.x;
+      v[1] = x.y;
+      v[2] = x.z;
+      v[3] = x.w;
+    } else {
+      const float4 x = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(p.G32) + g * plane + at);
+      v[0] = x.x;
+      v[1] = x.y;
+      v[2] = x.z;
+      v[3] = x.w;
+    }
+    const double cr = p.coef[g], ci = p.coef[p.G + g];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      re[j] = fma(cr, v[j], re[j]);
+      im[j] = fma(ci, v[j], im[j]);
+    }
   }
-  int e = 0;
-  if (p.expA) e += p.expA[(int64_t)b * p.M + m];
-  if (p.expB) e += p.expB[(int64_t)b * p.N + n];
-  const int64_t off = p.bC[b] + p.rowC[m] + p.colC[n];
-  if (p.c_complex) store_elem<double, true>(p.C, off, Cplx<double>{ldexp(re, e), ldexp(im, e)});
-  else store_elem<double, false>(p.C, off, Cplx<double>{ldexp(re, e), 0.0});
+  const int ea = p.expA ? p.expA[(int64_t)b * p.M + m] : 0;
+  const int64_t rb = p.bC[b] + p.rowC[m];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = n0 + j;
+    if (n >= p.N) break;
+    const int e = ea + (p.expB ? p.expB[(int64_t)b * p.N + n] : 0);
+    const int64_t off = rb + p.colC[n];
+    if (p.c_complex) store_elem<double, true>(p.C, off, Cplx<double>{ldexp(re[j], e), ldexp(im[j], e)});
+    else store_elem<double, false>(p.C, off, Cplx<double>{ldexp(re[j], e), 0.0});
+  }
 }
 
 // ------------------------------------------------------------------ host helpers
@@ -880,10 +905,10 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
                  (int32_t)d.N,
                  p.split_mode == MODE_I8,
                  ca || cb,
-                 FastDiv((uint32_t)std::max<int64_t>(1, std::min<int64_t>(d.N, UINT32_MAX))),
+                 FastDiv((uint32_t)std::max<int64_t>(1, std::min<int64_t>((d.N + 3) / 4, UINT32_MAX))),
                  FastDiv((uint32_t)std::max<int64_t>(1, std::min<int64_t>(d.M, UINT32_MAX))),
-                 d.B * d.M * d.N < (int64_t(1) << 32)};
-  const int64_t total = d.B * d.M * d.N;
+                 d.B * d.M * ((d.N + 3) / 4) < (int64_t(1) << 32)};
+  const int64_t total = d.B * d.M * ((d.N + 3) / 4);
   hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0, s, cm);
 }
 
