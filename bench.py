@@ -496,7 +496,22 @@ def main() -> int:
         A1, ia1, B1, ib1, dims, ic1 = wl.config1_inputs(0)
         a1, b1 = put(A1, ia1, dev), put(B1, ib1, dev)
         m1, c1, inf1 = contraction_case(a1, b1, {"dims": dims}, plain, 200, 10)
-        extra["c1_c64"] = {"us_per_call": round(m1 * 1e3, 2), "tflops": round(inf1["flops"] / (m1 * 1e-3) / 1e12, 4)}
+        # the same contraction through the C ABI alone: descriptors built once,
+        # the output preallocated, one muscle_hip_binary_einsum call per step
+        import ctypes
+
+        from muscle import _lib
+        from muscle.ops import _descs, _handle_for
+        h = _handle_for(dev)
+        dc, da, db = _descs(c1, a1, b1)
+        fn = _lib.lib.muscle_hip_binary_einsum
+
+        def abi_step():
+            fn(h, dc.ref, da.ref, db.ref, 0, 1e-12, None)
+        m1a, _ = timed(abi_step, 500, 20)
+        extra["c1_c64"] = {"us_per_call": round(m1 * 1e3, 2), "us_per_call_abi": round(m1a * 1e3, 2),
+                           "tflops": round(inf1["flops"] / (m1 * 1e-3) / 1e12, 4),
+                           "tflops_abi": round(inf1["flops"] / (m1a * 1e-3) / 1e12, 4)}
     if want("ladder_c1") and not args.no_cpu_baseline:
         from oracle import backend_base as bb
         res = {}
