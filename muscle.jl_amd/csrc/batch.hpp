@@ -35,6 +35,7 @@ struct BatchArgs {
   int32_t* counter;   // work queue head
   int64_t nitems;
   int32_t complex3m;
+  int32_t item_off;   // byte offset of the broadcast work item in dynamic LDS
 };
 
 // Launches the persistent kernel (grid = workgroups) preceded by a one-thread
@@ -43,6 +44,10 @@ struct BatchArgs {
 void launch_batch(bool dbl, bool cplx, const BatchArgs& a, int workgroups, size_t lds_bytes, hipStream_t s);
 // Dynamic LDS one workgroup needs for a tile shape.
 size_t batch_lds_bytes(bool dbl, bool cplx, int tile);
+// Where the work-item broadcast word goes in a launch's dynamic LDS of
+// lds_bytes (grown by 16 bytes when it goes at the end), so that two
+// workgroups still fit a CU's 160 KiB.
+size_t batch_item_offset(bool dbl, bool cplx, size_t& lds_bytes);
 // Persistent workgroups per CU the kernel is built for.
 constexpr int kBatchWgPerCu = 2;
 

@@ -260,7 +260,9 @@ void batch_launch(Batch& b, hipStream_t s) {
   a.counter = b.dcounter;
   a.nitems = (int64_t)b.hitems.size();
   a.complex3m = b.h->complex_mult == 3;
-  launch_batch(is_double(b.dtype), is_complex(b.dtype), a, b.workgroups, std::max<size_t>(b.lds, 16), s);
+  size_t lds = b.lds;
+  a.item_off = (int32_t)batch_item_offset(is_double(b.dtype), is_complex(b.dtype), lds);
+  launch_batch(is_double(b.dtype), is_complex(b.dtype), a, b.workgroups, lds, s);
 }
 
 void batch_run(Batch& b, int mode) {
