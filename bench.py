@@ -347,6 +347,54 @@ def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, thre
     return res
 
 
+def bench_simple_update(dev, chis, threads: int):
+    """Two-site gate updates of an MPS bond (physical dimension 2, ComplexF64): the
+    two contractions, the thin SVD of the (2 chi) x (2 chi) theta, the cut to chi,
+    normalization and AbsorbEqually; GPU time per update and the oracle's on the
+    host (numpy LAPACK, `threads` threads) for the same update."""
+    import numpy as np
+    import torch
+
+    import muscle
+    I = muscle.Index
+    res = {}
+    for chi in chis:
+        rng = np.random.default_rng(chi)
+        a = (rng.standard_normal((chi, 2, chi)) + 1j * rng.standard_normal((chi, 2, chi))) / np.sqrt(2 * chi)
+        b = (rng.standard_normal((chi, 2, chi)) + 1j * rng.standard_normal((chi, 2, chi))) / np.sqrt(2 * chi)
+        g = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0].reshape(2, 2, 2, 2,
+                                                                                                      order="F")
+        ia, ib, ig = ["l", "pa", "bond"], ["bond", "pb", "r"], ["pa", "pb", "ga", "gb"]
+        A, B, G = put(a, ia, dev), put(b, ib, dev), put(g, ig, dev)
+
+        def step():
+            return muscle.simple_update(A, I("pa"), B, I("pb"), I("bond"), G, I("ga"), I("gb"), normalize=True,
+                                        absorb=muscle.AbsorbEqually(), maxdim=chi)
+        n = 20 if chi <= 256 else 5
+        ms, (U, V) = timed(step, n, 2)
+        theta = muscle.binary_einsum(muscle.binary_einsum(A, B, dims=[I("bond")]), G, dims=[I("pa"), I("pb")])
+        ms_svd, _ = timed(lambda: muscle.tensor_svd_thin(theta, inds_u=[I("l"), I("ga")]), n, 1)
+        row = {"ms_per_update": round(ms, 3), "ms_svd": round(ms_svd, 3), "theta": f"{2 * chi}x{2 * chi}"}
+        if threads and chi <= 256:
+            from oracle import factor as of
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                (Uo, iu), (Vo, iv) = of.simple_update_base(a, ia, "pa", b, ib, "pb", "bond", g, ig, "ga", "gb",
+                                                           normalize=True, absorb="equally", maxdim=chi)
+            t_cpu = (time.perf_counter() - t0) / reps
+            from oracle import backend_base as bb
+            out = ["l", "pa", "pb", "r"]
+            ref = bb.binary_einsum_general(out, Uo, iu, Vo, iv)
+            got = bb.binary_einsum_general(out, U.parent().cpu().numpy(), iu, V.parent().cpu().numpy(), iv)
+            row.update({"cpu_oracle_ms": round(t_cpu * 1e3, 2), "speedup_vs_cpu": round(t_cpu * 1e3 / ms, 2),
+                        "rel_err_vs_cpu_oracle": rel(got, ref)})
+        res[f"chi{chi}"] = row
+        del A, B, G, U, V, theta
+    return res
+
+
 def measured_traffic(precision: str):
     """HBM bytes per launch of the headline kernel, from the committed rocprofv3
     FETCH_SIZE + WRITE_SIZE summary (profiles/traffic_config2_c64.txt), or None."""
@@ -369,7 +417,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only", default="all",
-                    help="comma list of: ladder_c2,c2_c32,c2h_c64,dims2,c1_c64,ladder_c1,batched,served "
+                    help="comma list of: ladder_c2,c2_c32,c2h_c64,dims2,c1_c64,ladder_c1,batched,served,simple_update "
                          "(the headline always runs)")
     ap.add_argument("--precision", default="native", help="arithmetic of the headline workload")
     ap.add_argument("--target", type=float, default=1e-10)
@@ -550,6 +598,15 @@ def main() -> int:
                 extra[f"served_{dist}"] = {"error": repr(e)}
                 log(f"served {dist} failed: {e!r}")
             torch.cuda.empty_cache()
+
+    if want("simple_update"):
+        try:
+            extra["simple_update_c64"] = bench_simple_update(dev, [64, 256, 1024], 0 if args.no_cpu_baseline
+                                                             else threads)
+            log(f"simple_update: {extra['simple_update_c64']}")
+        except Exception as e:  # reported in the JSON, not hidden
+            extra["simple_update_c64"] = {"error": repr(e)}
+            log(f"simple_update failed: {e!r}")
 
     peak = PEAK_TFLOPS["fp64"]
     out = {

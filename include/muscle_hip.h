@@ -183,6 +183,35 @@ MUSCLE_HIP_API int muscle_hip_unary_einsum(muscle_hip_handle_t handle,
                                            const muscle_hip_tensor_t* x,
                                            muscle_hip_info_t* info);
 
+/* --- factorization (the simple_update path) ----------------------------- */
+
+/* Thin SVD of A viewed as the matrix (U's row modes) x (V's row modes):
+ *   A[u..., v...] = sum_s U[u..., s] * S[s] * V[v..., s]
+ * U's modes are its row modes (a subset of A's, in the order that makes them
+ * the matrix rows, first fastest) followed by the bond mode s; V's likewise;
+ * S has the single mode s, of extent min(rows, columns), singular values in
+ * decreasing order.  V holds the conjugated right singular vectors, as the
+ * reference's Vt.  U and V have A's element type, S its real type.
+ * Replaces tensor_svd_thin(::Backend, A; inds_u, inds_v, ind_s)
+ * (src/Operations/tensor_svd.jl:100-125; Julia LinearAlgebra.svd); the matrix
+ * SVD is rocSOLVER's gesvd, the permutations are this library's copies.
+ * Synchronizes the handle's stream once (the solver's convergence flag). */
+MUSCLE_HIP_API int muscle_hip_svd_thin(muscle_hip_handle_t handle, const muscle_hip_tensor_t* u,
+                                       const muscle_hip_tensor_t* s, const muscle_hip_tensor_t* v,
+                                       const muscle_hip_tensor_t* a);
+
+/* Thin QR of A viewed as the matrix (Q's row modes) x (R's column modes):
+ *   A[q..., r...] = sum_v Q[q..., v] * R[v, r...]
+ * Q's modes are its row modes followed by the bond mode v; R's are v followed
+ * by its column modes; v has extent min(rows, columns).  Q has orthonormal
+ * columns; R is upper trapezoidal (LAPACK's Householder sign convention, the
+ * one Julia's qr uses).  Q and R have A's element type.
+ * Replaces tensor_qr_thin(::Backend, A; inds_q, inds_r, ind_virtual)
+ * (src/Operations/tensor_qr.jl:45-68; Julia LinearAlgebra.qr); the matrix QR
+ * is rocSOLVER's geqrf + orgqr/ungqr.  Enqueues on the handle's stream. */
+MUSCLE_HIP_API int muscle_hip_qr_thin(muscle_hip_handle_t handle, const muscle_hip_tensor_t* q,
+                                      const muscle_hip_tensor_t* r, const muscle_hip_tensor_t* a);
+
 /* --- batched contractions (one persistent launch) ---------------------- */
 
 typedef struct muscle_hip_batch_s* muscle_hip_batch_t;

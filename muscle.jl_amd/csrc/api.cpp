@@ -86,6 +86,7 @@ Handle::~Handle() {
   if (ws) (void)hipFree(ws);
   cache.clear();  // plans free their device tables
   release_lt(*this);
+  release_rb(*this);
 }
 
 Plan* Handle::find_plan(const std::string& key) {
@@ -606,6 +607,28 @@ int muscle_hip_unary_einsum(muscle_hip_handle_t handle, const muscle_hip_tensor_
     Plan* p = get_plan(*h, 2, MUSCLE_HIP_PRECISION_NATIVE, 0.0, y, x, nullptr);
     run_contraction(*h, *p, *y, *x, nullptr);
     fill_info(info, *p, y, x, nullptr);
+  });
+}
+
+int muscle_hip_svd_thin(muscle_hip_handle_t handle, const muscle_hip_tensor_t* u, const muscle_hip_tensor_t* s,
+                        const muscle_hip_tensor_t* v, const muscle_hip_tensor_t* a) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    if (!u || !s || !v || !a) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null tensor descriptor");
+    std::lock_guard<std::mutex> lk(h->mu);
+    svd_thin(*h, *u, *s, *v, *a);
+  });
+}
+
+int muscle_hip_qr_thin(muscle_hip_handle_t handle, const muscle_hip_tensor_t* q, const muscle_hip_tensor_t* r,
+                       const muscle_hip_tensor_t* a) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    if (!q || !r || !a) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null tensor descriptor");
+    std::lock_guard<std::mutex> lk(h->mu);
+    qr_thin(*h, *q, *r, *a);
   });
 }
 

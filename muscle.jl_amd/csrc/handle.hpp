@@ -94,6 +94,9 @@ struct Handle {
   int complex_mult = 3;  // MUSCLE_HIP_OPTION_COMPLEX_MULT
   void* lt = nullptr;     // hipBLASLt handle (created on first use)
   void* lt_ws = nullptr;  // its workspace (kLtWorkspace bytes)
+  void* rb = nullptr;     // rocBLAS handle of the factorizations (created on first use)
+  void* fws = nullptr;    // factorization workspace (matrix copies), grown like ws
+  size_t fws_size = 0;
   static constexpr int kTuneClasses = 25;
   int tune[kTuneClasses] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
                             -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};  // batched tile per size class
@@ -112,5 +115,10 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
                      const muscle_hip_tensor_t* b);
 Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
                const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b);
+// Thin SVD (factor.cpp) and the release of its rocBLAS handle and workspace.
+void svd_thin(Handle& h, const muscle_hip_tensor_t& U, const muscle_hip_tensor_t& S, const muscle_hip_tensor_t& V,
+              const muscle_hip_tensor_t& A);
+void qr_thin(Handle& h, const muscle_hip_tensor_t& Q, const muscle_hip_tensor_t& R, const muscle_hip_tensor_t& A);
+void release_rb(Handle& h);
 
 }  // namespace mh

@@ -120,4 +120,8 @@ struct PackOpArgs {
 };
 void launch_pack_operand(int elem_bytes, const PackOpArgs& a, hipStream_t s);
 
+// dst (k x n, ld k) = upper trapezoid of the first k rows of src (m x n, ld m)
+void launch_upper_trapezoid(int elem_bytes, const void* src, void* dst, int64_t m, int64_t k, int64_t n,
+                            hipStream_t s);
+
 }  // namespace mh

@@ -15,6 +15,12 @@ backend methods the front ends call:
   unary_einsum(::BackendHIP, inds_y, x)       (src/Operations/unary_einsum.jl:35)
   unary_einsum!(::BackendHIP, y, x)           (:44)
 
+and, for the two-site gate update that follows the contractions, the
+factorization behind it:
+
+  tensor_svd_thin(::BackendHIP, A; inds_u, inds_v, ind_s)   (src/Operations/tensor_svd.jl:59-68)
+  simple_update(::BackendHIP, A, ...)                        (src/Operations/simple_update.jl:37-87)
+
 It assumes `struct BackendHIP <: Backend end` next to the other backends
 (src/Backend.jl:6-14) and `struct DomainROCm <: Domain end` next to the other
 domains (src/Domain.jl:6-9).  Set ENV["MUSCLE_HIP_LIB"] to the library path.
@@ -167,6 +173,67 @@ function Muscle.unary_einsum(::BackendHIP, inds_y, x::Tensor)
     sizes = Dict(zip(inds(x), size(parent(x))))
     y = Tensor(similar(parent(x), Tuple(sizes[i] for i in inds_y)), collect(inds_y))
     return Muscle.unary_einsum!(BackendHIP(), y, x)
+end
+
+# --- factorization ---------------------------------------------------------------
+
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_svd_thin), ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(Muscle.simple_update), ::DomainROCm, ::DomainROCm, ::DomainROCm) = BackendHIP()
+
+# U[inds_u..., ind_s], s[ind_s], Vt[inds_v..., ind_s] with A = U s Vt, the
+# layout of BackendBase's tensor_svd_thin (src/Operations/tensor_svd.jl:100-125)
+function Muscle.tensor_svd_thin(::BackendHIP, A::Tensor; inds_u=(), inds_v=(), ind_s=Index(gensym(:vind)), kwargs...)
+    inds_u, inds_v = Muscle.factorinds(inds(A), inds_u, inds_v)
+    isdisjoint(inds_u, inds_v) || throw(ArgumentError("inds_u and inds_v must be disjoint"))
+    issetequal(inds_u ∪ inds_v, inds(A)) || throw(ArgumentError("inds_u and inds_v must cover inds(A)"))
+    ind_s ∉ inds(A) || throw(ArgumentError("ind_s is already an index of A"))
+    su = [size(A, i) for i in inds_u]
+    sv = [size(A, i) for i in inds_v]
+    k = min(prod(su), prod(sv))
+    T = eltype(parent(A))
+    U = Tensor(similar(parent(A), T, su..., k), [inds_u; ind_s])
+    s = Tensor(similar(parent(A), real(T), k), [ind_s])
+    Vt = Tensor(similar(parent(A), T, sv..., k), [inds_v; ind_s])
+    labels = modemap(U, s, Vt, A)
+    du, ku = describe(U, labels)
+    ds, ks = describe(s, labels)
+    dv, kv = describe(Vt, labels)
+    da, ka = describe(A, labels)
+    GC.@preserve U s Vt A ku ks kv ka begin
+        check(ccall((:muscle_hip_svd_thin, LIB), Cint,
+                    (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}),
+                    handle(), du, ds, dv, da))
+    end
+    return U, s, Vt
+end
+
+# simple_update.jl:37-87 on device tensors: the contractions and the absorption
+# run through this backend's binary_einsum and hadamard!, the SVD through
+# muscle_hip_svd_thin
+function Muscle.simple_update(::BackendHIP, A::Tensor, ind_physical_a::Index, B::Tensor, ind_physical_b::Index,
+                              ind_bond_ab::Index, G::Tensor, ind_physical_g_a::Index, ind_physical_g_b::Index;
+                              normalize::Bool=false, absorb::Muscle.AbsorbBehavior=Muscle.DontAbsorb(),
+                              atol::Float64=0.0, rtol::Float64=0.0, maxdim=nothing)
+    Θ = Muscle.binary_einsum(Muscle.binary_einsum(A, B; dims=[ind_bond_ab]), G; dims=[ind_physical_a, ind_physical_b])
+    Θ = replace(Θ, ind_physical_g_a => ind_physical_a, ind_physical_g_b => ind_physical_b)
+    U, S, V = Muscle.tensor_svd_thin(BackendHIP(), Θ; inds_u=setdiff(inds(A), [ind_bond_ab]),
+                                     inds_v=setdiff(inds(B), [ind_bond_ab]), ind_s=ind_bond_ab)
+    if !isnothing(maxdim)
+        χ = min(maxdim, length(S))
+        U, S, V = view(U, ind_bond_ab => 1:χ), view(S, ind_bond_ab => 1:χ), view(V, ind_bond_ab => 1:χ)
+    end
+    normalize && (parent(S) ./= sqrt(sum(abs2, parent(S))))
+    absorb isa Muscle.DontAbsorb && return U, S, V
+    if absorb isa Muscle.AbsorbU
+        U = Muscle.hadamard!(U, U, S)
+    elseif absorb isa Muscle.AbsorbV
+        V = Muscle.hadamard!(V, V, S)
+    elseif absorb isa Muscle.AbsorbEqually
+        S_sqrt = Tensor(sqrt.(parent(S)), inds(S))
+        U = Muscle.hadamard!(U, U, S_sqrt)
+        V = Muscle.hadamard!(V, V, S_sqrt)
+    end
+    return U, V
 end
 
 end # module

@@ -16,9 +16,12 @@ from .ops import (binary_einsum, binary_einsum_, hadamard, hadamard_, last_info,
                   unary_einsum_)
 from .tensor import Index, Tensor, col_major_strides, empty_col_major, inds, parent
 from .describe import describe
+from .factor import (AbsorbBehavior, AbsorbEqually, AbsorbU, AbsorbV, DontAbsorb, factorinds, simple_update,
+                     tensor_qr_thin, tensor_qr_thin_, tensor_svd_thin, tensor_svd_thin_, tensor_svd_trunc)
 
 __all__ = [
-    "Tensor", "Index", "inds", "parent", "binary_einsum", "binary_einsum_", "hadamard", "hadamard_",
+    "tensor_svd_thin", "tensor_svd_thin_", "tensor_svd_trunc", "tensor_qr_thin", "tensor_qr_thin_", "simple_update", "factorinds", "AbsorbBehavior", "DontAbsorb", "AbsorbU", "AbsorbV",
+    "AbsorbEqually", "Tensor", "Index", "inds", "parent", "binary_einsum", "binary_einsum_", "hadamard", "hadamard_",
     "unary_einsum", "unary_einsum_", "nonunique", "Backend", "BackendBase", "BackendHIP", "BackendOMEinsum",
     "BackendDefault", "BackendStrided", "BackendCuTENSOR", "BackendReactant", "BackendDagger", "Domain",
     "DomainHost", "DomainROCm", "domain", "with_backend", "choose_backend", "choose_backend_rule",

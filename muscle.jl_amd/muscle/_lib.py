@@ -157,6 +157,8 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_server_get_stats": ([ctypes.c_void_p, ctypes.POINTER(ServerStats)], ctypes.c_int),
         "muscle_hip_server_clock": ([ctypes.c_void_p], ctypes.c_double),
         "muscle_hip_server_destroy": ([ctypes.c_void_p], ctypes.c_int),
+        "muscle_hip_svd_thin": ([ctypes.c_void_p] + [ctypes.POINTER(TensorDesc)] * 4, ctypes.c_int),
+    "muscle_hip_qr_thin": ([ctypes.c_void_p] + [ctypes.POINTER(TensorDesc)] * 3, ctypes.c_int),
         "muscle_hip_describe": ([T, T, T, ctypes.c_int, ctypes.c_double, I, ctypes.POINTER(ctypes.c_int32),
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     }
