@@ -6,16 +6,32 @@
 // permutes A to (inds_u..., inds_v...), reshapes it to an m x n matrix,
 // calls LAPACK's SVD and reshapes U (m x k), s (k) and conj(V) (n x k) back
 // to tensors.  Here the permutation copies are this library's own unary
-// (copy) plans, and the SVD of the m x n matrix is rocSOLVER's gesvd on the
-// handle's stream; U and V^H come out as the column-major m x k and k x n
-// matrices that the output tensors then view.  tensor_qr_thin
-// (src/Operations/tensor_qr.jl:45-68) is the same with geqrf + orgqr/ungqr;
-// R is the upper trapezoid of the first k rows of the factored matrix.
+// (copy) plans.  The SVD of the m x n matrix M comes from the Hermitian
+// eigenproblem of the augmented matrix [[0, M], [M^H, 0]] (rocSOLVER
+// syevd/heevd, divide and conquer): its eigenpairs are (+-sigma_i,
+// [u_i; +-v_i] / sqrt(2)), so the top k eigenvectors give U and V after each
+// half is normalised.  The eigensolver is backward stable, so the singular
+// values carry LAPACK's absolute accuracy (a few eps * ||M||) and the vectors
+// its eps * ||M|| / gap accuracy.  Two passes of Cholesky QR then make U and V
+// orthonormal to working precision (column j is only corrected with columns
+// of larger singular value, so U s V^H moves by O(eps * ||M||)); singular
+// values at the eigensolver's noise level (where +sigma and -sigma pairs and
+// the null space mix) get a deterministic pseudo-random orthonormal
+// completion instead, as LAPACK's vectors for a zero singular value are any
+// orthonormal completion too.  A matrix more than twice as tall as wide is
+// first reduced by QR; a wide one is factored as its transpose.  The eigen
+// route replaces rocSOLVER's gesvd, whose bidiagonal QR iteration is about
+// 60x slower at 512 x 512 on this GPU.
+//
+// tensor_qr_thin (src/Operations/tensor_qr.jl:45-68) is the same permutation
+// with geqrf + orgqr/ungqr; R is the upper trapezoid of the first k rows of
+// the factored matrix.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -66,24 +82,103 @@ void ensure_fws(Handle& h, size_t bytes) {
 
 size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
-rocblas_status gesvd(int dt, rocblas_handle rb, int m, int n, void* A, void* S, void* U, void* V, void* E,
-                     rocblas_int* info) {
-  const int k = std::min(m, n);
+rocblas_status heevd(int dt, rocblas_handle rb, int n, void* A, void* w, void* e, rocblas_int* info) {
+  const rocblas_evect ev = rocblas_evect_original;
+  const rocblas_fill up = rocblas_fill_upper;
   switch (dt) {
     case MUSCLE_HIP_F32:
-      return rocsolver_sgesvd(rb, rocblas_svect_singular, rocblas_svect_singular, m, n, (float*)A, m, (float*)S,
-                              (float*)U, m, (float*)V, k, (float*)E, rocblas_outofplace, info);
+      return rocsolver_ssyevd(rb, ev, up, n, (float*)A, n, (float*)w, (float*)e, info);
     case MUSCLE_HIP_F64:
-      return rocsolver_dgesvd(rb, rocblas_svect_singular, rocblas_svect_singular, m, n, (double*)A, m, (double*)S,
-                              (double*)U, m, (double*)V, k, (double*)E, rocblas_outofplace, info);
+      return rocsolver_dsyevd(rb, ev, up, n, (double*)A, n, (double*)w, (double*)e, info);
     case MUSCLE_HIP_C32:
-      return rocsolver_cgesvd(rb, rocblas_svect_singular, rocblas_svect_singular, m, n, (rocblas_float_complex*)A,
-                              m, (float*)S, (rocblas_float_complex*)U, m, (rocblas_float_complex*)V, k, (float*)E,
-                              rocblas_outofplace, info);
+      return rocsolver_cheevd(rb, ev, up, n, (rocblas_float_complex*)A, n, (float*)w, (float*)e, info);
     default:
-      return rocsolver_zgesvd(rb, rocblas_svect_singular, rocblas_svect_singular, m, n, (rocblas_double_complex*)A,
-                              m, (double*)S, (rocblas_double_complex*)U, m, (rocblas_double_complex*)V, k,
-                              (double*)E, rocblas_outofplace, info);
+      return rocsolver_zheevd(rb, ev, up, n, (rocblas_double_complex*)A, n, (double*)w, (double*)e, info);
+  }
+}
+
+// C = alpha op(A) op(B) + beta C, op the identity or the conjugate transpose
+rocblas_status gemm(int dt, rocblas_handle rb, bool ha, bool hb, int64_t m, int64_t n, int64_t k, double alpha,
+                    const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc) {
+  const bool cplx = dt == MUSCLE_HIP_C32 || dt == MUSCLE_HIP_C64;
+  const rocblas_operation h = cplx ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+  const rocblas_operation oa = ha ? h : rocblas_operation_none, ob = hb ? h : rocblas_operation_none;
+  switch (dt) {
+    case MUSCLE_HIP_F32: {
+      const float a = (float)alpha, b = (float)beta;
+      return rocblas_sgemm(rb, oa, ob, (int)m, (int)n, (int)k, &a, (const float*)A, (int)lda, (const float*)B,
+                           (int)ldb, &b, (float*)C, (int)ldc);
+    }
+    case MUSCLE_HIP_F64:
+      return rocblas_dgemm(rb, oa, ob, (int)m, (int)n, (int)k, &alpha, (const double*)A, (int)lda, (const double*)B,
+                           (int)ldb, &beta, (double*)C, (int)ldc);
+    case MUSCLE_HIP_C32: {
+      const rocblas_float_complex a((float)alpha, 0.f), b((float)beta, 0.f);
+      return rocblas_cgemm(rb, oa, ob, (int)m, (int)n, (int)k, &a, (const rocblas_float_complex*)A, (int)lda,
+                           (const rocblas_float_complex*)B, (int)ldb, &b, (rocblas_float_complex*)C, (int)ldc);
+    }
+    default: {
+      const rocblas_double_complex a(alpha, 0.0), b(beta, 0.0);
+      return rocblas_zgemm(rb, oa, ob, (int)m, (int)n, (int)k, &a, (const rocblas_double_complex*)A, (int)lda,
+                           (const rocblas_double_complex*)B, (int)ldb, &b, (rocblas_double_complex*)C, (int)ldc);
+    }
+  }
+}
+
+// G (n x n, upper) = R with R^H R = G
+rocblas_status potrf(int dt, rocblas_handle rb, int64_t n, void* G, rocblas_int* info) {
+  const rocblas_fill up = rocblas_fill_upper;
+  switch (dt) {
+    case MUSCLE_HIP_F32:
+      return rocsolver_spotrf(rb, up, (int)n, (float*)G, (int)n, info);
+    case MUSCLE_HIP_F64:
+      return rocsolver_dpotrf(rb, up, (int)n, (double*)G, (int)n, info);
+    case MUSCLE_HIP_C32:
+      return rocsolver_cpotrf(rb, up, (int)n, (rocblas_float_complex*)G, (int)n, info);
+    default:
+      return rocsolver_zpotrf(rb, up, (int)n, (rocblas_double_complex*)G, (int)n, info);
+  }
+}
+
+// X (m x n) = X R^{-1}, R (n x n) upper triangular
+rocblas_status trsm_right_upper(int dt, rocblas_handle rb, int64_t m, int64_t n, const void* R, void* X) {
+  const rocblas_side sd = rocblas_side_right;
+  const rocblas_fill up = rocblas_fill_upper;
+  const rocblas_operation no = rocblas_operation_none;
+  const rocblas_diagonal dg = rocblas_diagonal_non_unit;
+  switch (dt) {
+    case MUSCLE_HIP_F32: {
+      const float one = 1.f;
+      return rocblas_strsm(rb, sd, up, no, dg, (int)m, (int)n, &one, (const float*)R, (int)n, (float*)X, (int)m);
+    }
+    case MUSCLE_HIP_F64: {
+      const double one = 1.0;
+      return rocblas_dtrsm(rb, sd, up, no, dg, (int)m, (int)n, &one, (const double*)R, (int)n, (double*)X, (int)m);
+    }
+    case MUSCLE_HIP_C32: {
+      const rocblas_float_complex one(1.f, 0.f);
+      return rocblas_ctrsm(rb, sd, up, no, dg, (int)m, (int)n, &one, (const rocblas_float_complex*)R, (int)n,
+                           (rocblas_float_complex*)X, (int)m);
+    }
+    default: {
+      const rocblas_double_complex one(1.0, 0.0);
+      return rocblas_ztrsm(rb, sd, up, no, dg, (int)m, (int)n, &one, (const rocblas_double_complex*)R, (int)n,
+                           (rocblas_double_complex*)X, (int)m);
+    }
+  }
+}
+
+void solver_check(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success) fail(MUSCLE_HIP_HIP_ERROR, std::string("rocSOLVER/rocBLAS ") + what + " failed");
+}
+
+// Two passes of Cholesky QR over the columns of X (rows x cols, ld rows):
+// X = X R^{-1} with R^H R = X^H X.  info[0..1] receive potrf's flags.
+void orthonormalize(int dt, rocblas_handle rb, void* X, int64_t rows, int64_t cols, void* G, rocblas_int* info) {
+  for (int pass = 0; pass < 2; ++pass) {
+    solver_check(gemm(dt, rb, true, false, cols, cols, rows, 1.0, X, rows, X, rows, 0.0, G, cols), "gemm");
+    solver_check(potrf(dt, rb, cols, G, info + pass), "potrf");
+    solver_check(trsm_right_upper(dt, rb, rows, cols, G, X), "trsm");
   }
 }
 
@@ -193,66 +288,134 @@ void svd_thin(Handle& h, const muscle_hip_tensor_t& U, const muscle_hip_tensor_t
   const int32_t sm = S.modes[0];
   if (U.modes[U.rank - 1] != sm || V.modes[V.rank - 1] != sm)
     fail(MUSCLE_HIP_INVALID_ARGUMENT, "svd: the bond mode must be the last mode of U and of V and the mode of S");
-  if (A.dtype != U.dtype || A.dtype != V.dtype || S.dtype != (is_double(A.dtype) ? MUSCLE_HIP_F64 : MUSCLE_HIP_F32))
+  const int dt = A.dtype;
+  if (dt != U.dtype || dt != V.dtype || S.dtype != (is_double(dt) ? MUSCLE_HIP_F64 : MUSCLE_HIP_F32))
     fail(MUSCLE_HIP_INVALID_ARGUMENT, "svd: U and V have A's element type, S its real type");
   // A's modes split into U's (u) and V's (v) row modes, each once
-  View mat, uview, vview, sview;
+  View mat;
   const int64_t m = add_rows(mat, A, U, 0, U.rank - 1, sm, "svd");
   const int64_t n = add_rows(mat, A, V, 0, V.rank - 1, sm, "svd");
   const int64_t k = std::min(m, n);
   if (U.extents[U.rank - 1] != k || V.extents[V.rank - 1] != k || S.extents[0] != k)
     fail(MUSCLE_HIP_DIM_MISMATCH, "svd: the bond extent must be min(rows of U, rows of V)");
-
-  const size_t es = dtype_size(A.dtype), rs = dtype_size(S.dtype);
-  const size_t o_mat = 0, o_u = o_mat + al((size_t)(m * n) * es), o_v = o_u + al((size_t)(m * k) * es),
-               o_s = o_v + al((size_t)(k * n) * es), o_e = o_s + al((size_t)k * rs), o_info = o_e + al((size_t)k * rs),
-               total = o_info + 256;
   finish_matrix(mat, A, m, n, nullptr, "svd");
   if (k == 0) return;
-  ensure_fws(h, total);
+
+  // Factor the tall orientation: M (m x n) when m >= n, else M^T, whose
+  // factors are M's with U and V exchanged (M^T = U' s V'^H gives
+  // M = conj(V') s conj(U')^H).  Either way the row-group tensor RT receives
+  // the staged left vectors and the column-group tensor CT the conjugated
+  // right ones.
+  const bool tall = m >= n;
+  const muscle_hip_tensor_t& RT = tall ? U : V;
+  const muscle_hip_tensor_t& CT = tall ? V : U;
+  const int64_t rows = tall ? m : n, cols = k;
+  const bool reduce = rows > 2 * cols;  // QR first, so that the eigenproblem is 2 cols wide
+  const int64_t top = reduce ? cols : rows, nev = top + cols;
+  if (nev >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "svd: matrix too large");
+  const size_t es = dtype_size(dt), rs = dtype_size(S.dtype);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off += al(bytes);
+    return o;
+  };
+  const size_t o_h = carve((size_t)(nev * nev) * es), o_w = carve((size_t)nev * rs), o_e = carve((size_t)nev * rs),
+               o_u = carve((size_t)(rows * k) * es), o_ut = reduce ? carve((size_t)(top * k) * es) : o_u,
+               o_v = carve((size_t)(cols * k) * es), o_g = carve((size_t)(k * k) * es),
+               o_q = reduce ? carve((size_t)(rows * cols) * es) : 0, o_tau = reduce ? carve((size_t)k * es) : 0,
+               o_s = carve((size_t)k * rs), o_info = carve(8 * sizeof(rocblas_int));
+  ensure_fws(h, off);
   char* w = (char*)h.fws;
   rocblas_handle rb = solver_on_stream(h);
+  rocblas_int* info = (rocblas_int*)(w + o_info);
 
-  // A -> column-major m x n matrix, rows = U's row modes (first fastest), columns = V's
-  mat.t.data = w + o_mat;
-  copy_tensor(h, mat.t, A);
-
-  const rocblas_status rs_ = gesvd(A.dtype, rb, (int)m, (int)n, w + o_mat, w + o_s, w + o_u,
-                                   w + o_v, w + o_e, (rocblas_int*)(w + o_info));
-  if (rs_ != rocblas_status_success) fail(MUSCLE_HIP_HIP_ERROR, "rocsolver gesvd failed to launch");
-  rocblas_int info = 0;
-  MH_HIP_CHECK(hipMemcpyAsync(&info, w + o_info, sizeof info, hipMemcpyDeviceToHost, h.stream));
-  MH_HIP_CHECK(hipStreamSynchronize(h.stream));
-  if (info != 0) fail(MUSCLE_HIP_HIP_ERROR, "svd: the bidiagonal QR iteration did not converge");
-
-  // U (m x k): modes (u..., s), strides column-major then m
-  uview.modes.assign(mat.modes.begin(), mat.modes.begin() + (U.rank - 1));
-  uview.ext.assign(mat.ext.begin(), mat.ext.begin() + (U.rank - 1));
-  uview.str.assign(mat.str.begin(), mat.str.begin() + (U.rank - 1));
-  uview.modes.push_back(sm);
-  uview.ext.push_back(k);
-  uview.str.push_back(m);
-  uview.finish(A.dtype, w + o_u);
-  copy_tensor(h, U, uview.t);
-  // V^H (k x n, leading dimension k): element (s, v) = conj(V[v, s]), which is
-  // exactly the reference's Vt[v..., s]
-  vview.modes.push_back(sm);
-  vview.ext.push_back(k);
-  vview.str.push_back(1);
-  int64_t sv = k;
-  for (int i = 0; i < V.rank - 1; ++i) {
-    vview.modes.push_back(V.modes[i]);
-    vview.ext.push_back(V.extents[i]);
-    vview.str.push_back(sv);
-    sv *= V.extents[i];
+  // the matrix, written by a copy plan straight into the augmented matrix's
+  // upper-right block (or into the QR buffer first); the rest of the
+  // augmented matrix is zero
+  View mv;
+  auto add_group = [&](const muscle_hip_tensor_t& T, int64_t stride) {
+    for (int i = 0; i < T.rank - 1; ++i) {
+      mv.modes.push_back(T.modes[i]);
+      mv.ext.push_back(T.extents[i]);
+      mv.str.push_back(stride);
+      stride *= T.extents[i];
+    }
+  };
+  add_group(RT, 1);
+  add_group(CT, reduce ? rows : nev);
+  launch_zero(w + o_h, (int64_t)((size_t)(nev * nev) * es), h.stream);
+  MH_HIP_CHECK(hipGetLastError());
+  if (!reduce) {
+    mv.finish(dt, w + o_h + (size_t)(rows * nev) * es);
+    copy_tensor(h, mv.t, A);
+  } else {
+    mv.finish(dt, w + o_q);
+    copy_tensor(h, mv.t, A);
+    solver_check(geqrf(dt, rb, (int)rows, (int)cols, w + o_q, w + o_tau), "geqrf");
+    launch_upper_trapezoid((int)es, w + o_q, w + o_h + (size_t)(top * nev) * es, rows, cols, cols, nev, h.stream);
+    MH_HIP_CHECK(hipGetLastError());
+    solver_check(orgqr(dt, rb, (int)rows, (int)cols, w + o_q, w + o_tau), "orgqr");
   }
-  vview.finish(A.dtype, w + o_v);
-  copy_tensor(h, V, vview.t);
-  sview.modes.push_back(sm);
-  sview.ext.push_back(k);
-  sview.str.push_back(1);
-  sview.finish(S.dtype, w + o_s);
-  copy_tensor(h, S, sview.t);
+  solver_check(heevd(dt, rb, (int)nev, w + o_h, w + o_w, w + o_e, info), "syevd/heevd");
+  launch_split_pairs(dt, w + o_h, w + o_w, nev, top, cols, k, w + o_ut, w + o_v, w + o_s, h.stream);
+  MH_HIP_CHECK(hipGetLastError());
+
+  // the eigenvalues decide which columns sit at the eigensolver's noise level
+  std::vector<char> wh((size_t)nev * rs);
+  rocblas_int info0 = 0;
+  MH_HIP_CHECK(hipMemcpyAsync(wh.data(), w + o_w, wh.size(), hipMemcpyDeviceToHost, h.stream));
+  MH_HIP_CHECK(hipMemcpyAsync(&info0, info, sizeof info0, hipMemcpyDeviceToHost, h.stream));
+  MH_HIP_CHECK(hipStreamSynchronize(h.stream));
+  if (info0 != 0) fail(MUSCLE_HIP_HIP_ERROR, "svd: the eigensolver did not converge");
+  auto lam = [&](int64_t i) -> double {  // i-th largest eigenvalue
+    const int64_t j = nev - 1 - i;
+    return rs == 8 ? ((const double*)wh.data())[j] : (double)((const float*)wh.data())[j];
+  };
+  const double eps = rs == 8 ? 0x1p-52 : 0x1p-23;
+  const double tol = 8.0 * std::sqrt((double)nev) * eps * std::max(lam(0), 0.0);
+  int64_t r = 0;
+  while (r < k && lam(r) > tol) ++r;
+  if (r < k) {
+    launch_random_unit(dt, w + o_ut + (size_t)(r * top) * es, (k - r) * top, 0x5EED0001ull, h.stream);
+    launch_random_unit(dt, w + o_v + (size_t)(r * cols) * es, (k - r) * cols, 0x5EED0002ull, h.stream);
+    MH_HIP_CHECK(hipGetLastError());
+  }
+  orthonormalize(dt, rb, w + o_ut, top, k, w + o_g, info + 1);
+  orthonormalize(dt, rb, w + o_v, cols, k, w + o_g, info + 3);
+  if (reduce)  // U = Q U_R
+    solver_check(gemm(dt, rb, false, false, rows, k, cols, 1.0, w + o_q, rows, w + o_ut, cols, 0.0, w + o_u, rows),
+                 "gemm");
+
+  // staged factors (column-major, bond last) -> the caller's layouts
+  View uv, vv, sv;
+  auto staged = [&](View& t, const muscle_hip_tensor_t& T, int64_t ld, size_t o) {
+    int64_t st = 1;
+    for (int i = 0; i < T.rank - 1; ++i) {
+      t.modes.push_back(T.modes[i]);
+      t.ext.push_back(T.extents[i]);
+      t.str.push_back(st);
+      st *= T.extents[i];
+    }
+    t.modes.push_back(sm);
+    t.ext.push_back(k);
+    t.str.push_back(ld);
+    t.finish(dt, w + o);
+  };
+  staged(uv, RT, rows, o_u);
+  staged(vv, CT, cols, o_v);
+  copy_tensor(h, RT, uv.t);
+  copy_tensor(h, CT, vv.t);
+  sv.modes.push_back(sm);
+  sv.ext.push_back(k);
+  sv.str.push_back(1);
+  sv.finish(S.dtype, w + o_s);
+  copy_tensor(h, S, sv.t);
+  rocblas_int infos[4] = {0, 0, 0, 0};
+  MH_HIP_CHECK(hipMemcpyAsync(infos, info + 1, sizeof infos, hipMemcpyDeviceToHost, h.stream));
+  MH_HIP_CHECK(hipStreamSynchronize(h.stream));
+  for (rocblas_int f : infos)
+    if (f != 0) fail(MUSCLE_HIP_HIP_ERROR, "svd: the singular vectors lost rank in the Cholesky QR");
 }
 
 // Q[q..., v], R[v, r...] from A[q ∪ r]: see muscle_hip_qr_thin.
@@ -282,10 +445,9 @@ void qr_thin(Handle& h, const muscle_hip_tensor_t& Q, const muscle_hip_tensor_t&
   rocblas_handle rb = solver_on_stream(h);
   mat.t.data = w + o_mat;
   copy_tensor(h, mat.t, A);
-  if (geqrf(A.dtype, rb, (int)m, (int)n, w + o_mat, w + o_tau) != rocblas_status_success)
-    fail(MUSCLE_HIP_HIP_ERROR, "rocsolver geqrf failed to launch");
+  solver_check(geqrf(A.dtype, rb, (int)m, (int)n, w + o_mat, w + o_tau), "geqrf");
   // R (k x n): the upper trapezoid, before orgqr overwrites the matrix
-  launch_upper_trapezoid((int)es, w + o_mat, w + o_r, m, k, n, h.stream);
+  launch_upper_trapezoid((int)es, w + o_mat, w + o_r, m, k, n, k, h.stream);
   MH_HIP_CHECK(hipGetLastError());
   rview.modes.push_back(bond);
   rview.ext.push_back(k);
@@ -299,8 +461,7 @@ void qr_thin(Handle& h, const muscle_hip_tensor_t& Q, const muscle_hip_tensor_t&
   }
   rview.finish(A.dtype, w + o_r);
   copy_tensor(h, R, rview.t);
-  if (orgqr(A.dtype, rb, (int)m, (int)k, w + o_mat, w + o_tau) != rocblas_status_success)
-    fail(MUSCLE_HIP_HIP_ERROR, "rocsolver orgqr failed to launch");
+  solver_check(orgqr(A.dtype, rb, (int)m, (int)k, w + o_mat, w + o_tau), "orgqr");
   // Q (m x k, ld m): modes (q..., v)
   qview.modes.assign(mat.modes.begin(), mat.modes.begin() + (Q.rank - 1));
   qview.ext.assign(mat.ext.begin(), mat.ext.begin() + (Q.rank - 1));

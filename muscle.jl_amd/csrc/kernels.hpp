@@ -120,8 +120,16 @@ struct PackOpArgs {
 };
 void launch_pack_operand(int elem_bytes, const PackOpArgs& a, hipStream_t s);
 
-// dst (k x n, ld k) = upper trapezoid of the first k rows of src (m x n, ld m)
-void launch_upper_trapezoid(int elem_bytes, const void* src, void* dst, int64_t m, int64_t k, int64_t n,
+// dst (k x n, ld ldd) = upper trapezoid of the first k rows of src (m x n, ld m)
+void launch_upper_trapezoid(int elem_bytes, const void* src, void* dst, int64_t m, int64_t k, int64_t n, int64_t ldd,
                             hipStream_t s);
+// zero `bytes` (a multiple of 4) at a 4-byte aligned dst
+void launch_zero(void* dst, int64_t bytes, hipStream_t s);
+// `count` elements of dtype with deterministic pseudo-random parts in [-1, 1)
+void launch_random_unit(int dtype, void* dst, int64_t count, uint64_t seed, hipStream_t s);
+// singular pairs from the top k eigenvectors of the nev x nev augmented matrix
+// (eigenvalues w ascending): U (top x k), Vc (bot x k, conjugated), s (k)
+void launch_split_pairs(int dtype, const void* H, const void* w, int64_t nev, int64_t top, int64_t bot, int64_t k,
+                        void* U, void* Vc, void* s, hipStream_t st);
 
 }  // namespace mh

@@ -78,6 +78,48 @@ def test_svd_thin_bond_256():
     check_svd(randn(rng, (256, 2, 2, 256)), ["l", "p", "q", "r"], inds_u=["l", "p"])
 
 
+def isometry_err(t, k):
+    m = t.reshape(-1, k, order="F")
+    return float(np.abs(m.conj().T @ m - np.eye(k)).max())
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64])
+def test_svd_thin_rank_deficient(dtype):
+    """Rank 8 in 64 x 48 (a null space the vectors must still span orthonormally)
+    and a spectrum falling from 1 to 1e-10 (vectors of tiny singular values)."""
+    rng = np.random.default_rng(31)
+    x, y = randn(rng, (64, 8), dtype), randn(rng, (8, 48), dtype)
+    for A_np in (x @ y, (x @ y).T.copy()):
+        A = dev_tensor(A_np, [I("a"), I("b")])
+        U, S, Vt = muscle.tensor_svd_thin(A, inds_u=[I("a")], ind_s=I("x"))
+        u, s, v = host(U), host(S), host(Vt)
+        so = np.linalg.svd(A_np, compute_uv=False)
+        assert np.abs(s - so).max() <= 1e-13 * so[0]
+        assert isometry_err(u, 48) <= 1e-12 and isometry_err(v, 48) <= 1e-12
+        assert rel_err((u * s) @ v.T, A_np) <= 1e-13
+    q1 = np.linalg.qr(randn(rng, (96, 80), dtype))[0]
+    q2 = np.linalg.qr(randn(rng, (80, 80), dtype))[0]
+    sig = 10.0 ** (-np.arange(80) / 8)
+    A_np = (q1 * sig) @ q2.conj().T
+    A = dev_tensor(A_np, [I("a"), I("b")])
+    U, S, Vt = muscle.tensor_svd_thin(A, inds_u=[I("a")], ind_s=I("x"))
+    u, s, v = host(U), host(S), host(Vt)
+    assert np.abs(s - sig).max() <= 1e-14
+    assert isometry_err(u, 80) <= 1e-12 and isometry_err(v, 80) <= 1e-12
+    assert rel_err((u * s) @ v.T, A_np) <= 1e-13
+    # the leading vectors (relative gaps ~ 0.25) match the exact ones up to a phase
+    for x_ in range(10):
+        assert abs(abs(np.vdot(q1[:, x_], u[:, x_])) - 1) <= 1e-12
+        assert abs(abs(np.vdot(q2[:, x_].conj(), v[:, x_])) - 1) <= 1e-12
+
+
+def test_svd_thin_zero_matrix():
+    A = dev_tensor(np.zeros((5, 3)), [I("a"), I("b")])
+    U, S, Vt = muscle.tensor_svd_thin(A, inds_u=[I("a")], ind_s=I("x"))
+    assert np.array_equal(host(S), np.zeros(3))
+    assert isometry_err(host(U), 3) <= 1e-14 and isometry_err(host(Vt), 3) <= 1e-14
+
+
 def test_svd_thin_argument_errors():
     A = dev_tensor(randn(np.random.default_rng(0), (2, 4, 6, 8)), [I(i) for i in "ijkl"])
     for kw in ({}, {"inds_u": [I("z")]}, {"inds_v": [I("z")]}, {"inds_u": A.inds()}, {"inds_v": A.inds()},
