@@ -52,7 +52,8 @@ struct ScaleArgs {
   const void* X;
   OffTab rowX, kX, bX;
   int32_t* exps;  // [B][R]
-  int32_t R, K, B, cplx, bias;
+  double* parts;  // [nparts][B][R] partial maxima (scale_part_kernel)
+  int32_t R, K, B, cplx, bias, kchunk;
 };
 
 // One wavefront per (batch, row): e = floor(log2(max_k |re| + |im|)) + bias.
@@ -67,6 +68,7 @@ __global__ __launch_bounds__(NT) void scale_kernel(ScaleArgs p) {
   const int b = (int)(row / p.R), r = (int)(row - (int64_t)b * p.R);
   const int64_t base = p.bX[b] + p.rowX[r];
   double mx = 0.0;
+#pragma unroll 4
   for (int k = lane; k < p.K; k += 64) {
     const Cplx<double> v = p.cplx ? load_elem<double, true>(p.X, base + p.kX[k])
                                   : load_elem<double, false>(p.X, base + p.kX[k]);
@@ -76,18 +78,53 @@ __global__ __launch_bounds__(NT) void scale_kernel(ScaleArgs p) {
   if (lane == 0) p.exps[row] = scale_exp(mx, p.bias);
 }
 
+// Row-contiguous operands: a workgroup takes 64 neighbouring rows (one per
+// lane, so every load instruction reads 64 neighbouring elements) and a
+// kchunk-long piece of K (the four wavefronts interleave over it), and writes
+// the rows' partial maxima; the pack kernel reduces the pieces.
+constexpr int SCALE_ROWS = 64, SCALE_KCHUNK = 256;
+
+__global__ __launch_bounds__(NT) void scale_part_kernel(ScaleArgs p) {
+  __shared__ double red[NT / 64][SCALE_ROWS];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = blockIdx.x * SCALE_ROWS + lane, b = blockIdx.z;
+  const int k0 = blockIdx.y * p.kchunk, k1 = min(p.K, k0 + p.kchunk);
+  double mx = 0.0;
+  if (r < p.R) {
+    const int64_t base = p.bX[b] + p.rowX[r];
+#pragma unroll 8
+    for (int k = k0 + w; k < k1; k += NT / 64) {
+      const Cplx<double> v = p.cplx ? load_elem<double, true>(p.X, base + p.kX[k])
+                                    : load_elem<double, false>(p.X, base + p.kX[k]);
+      mx = fmax(mx, fabs(v.re) + fabs(v.im));
+    }
+  }
+  red[w][lane] = mx;
+  __syncthreads();
+  if (w == 0 && r < p.R) {
+#pragma unroll
+    for (int j = 1; j < NT / 64; ++j) mx = fmax(mx, red[j][lane]);
+    p.parts[((int64_t)blockIdx.y * p.B + b) * p.R + r] = mx;
+  }
+}
+
 // ------------------------------------------------------------------ pack
 struct PackArgs {
   const void* X;
   OffTab rowX, kX, bX;
-  const int32_t* exps;  // [B][R] or null (no scaling)
+  int32_t* exps;        // [B][R] or null (no scaling)
+  const double* parts;  // [nparts][B][R] partial maxima: exps are made here
   void* Q;              // [planes][B][Rp][S][Kp] elements
   int32_t R, K, B, Rp, Kp;
   int32_t cplx, r_major, planes, S, mode;
   int32_t rev;          // slices stored in reverse order within a row
+  int32_t nparts, bias;
 };
 
-constexpr int PR = 32, PK = 64;  // pack tile: 32 rows x 64 k
+// pack tile: 128 bytes of K per row (one cache line per row and slice on
+// the way out), 2048 values: 16 rows x 128 k (int8) or 32 rows x 64 k (16-bit)
+constexpr int pack_k(int mode) { return mode == MODE_I8 ? 128 : 64; }
+constexpr int pack_r(int mode) { return 2048 / pack_k(mode); }
 
 // Slices of one value into nps bytes / halfwords (per plane, S slices).
 template <int MODE>
@@ -116,95 +153,113 @@ __device__ inline void slice_value(double res, int s_count, uint32_t* out, int s
   }
 }
 
-// One 32-row x 64-k tile: every thread gathers 2 x 4 consecutive-k values of
+// One PR-row x PK-k tile: every thread gathers 2 x 4 consecutive-k values of
 // one row (rows side by side for row-contiguous operands, k-quads side by
-// side otherwise), slices them, and stores each plane-slice's 4 values as one
-// (int8) or two (bf16/f16) 32-bit LDS words; rows are PK*ES/4 + 1 words apart,
-// which keeps those stores free of bank conflicts.  The tile then leaves as
-// 16-byte chunks of K-contiguous packed rows.
+// side otherwise) once, then, one plane at a time, slices them and stores each
+// slice's 4 values as one (int8) or two (bf16/f16) 32-bit LDS words; rows are
+// PK*ES/4 + 1 words apart, which keeps those stores free of bank conflicts.
+// The plane's tile then leaves as 16-byte chunks of K-contiguous packed rows.
+// Staging one plane at a time keeps the LDS tile small (occupancy).  With
+// partial maxima the tile's row exponents are reduced here first, and the
+// k-tile 0 blocks store them for the combine.
 template <int MODE>
 __global__ __launch_bounds__(NT) void pack_kernel(PackArgs p) {
   constexpr int ES = MODE == MODE_I8 ? 1 : 2;
-  constexpr int MAXPS = MODE == MODE_I8 ? 3 * 7 : (MODE == MODE_BF16 ? 3 * 3 : 3 * 2);
   constexpr int WPQ = ES;                 // LDS words per k-quad
+  constexpr int PK = pack_k(MODE), PR = pack_r(MODE);
   constexpr int RW = PK * ES / 4 + 1;     // LDS words per tile row
   constexpr int MAXS = MODE == MODE_I8 ? 7 : 3;
-  __shared__ uint32_t tile[MAXPS][PR][RW];
+  __shared__ uint32_t tile[MAXS][PR][RW];
+  __shared__ int32_t s_exp[PR];
   const int k0 = blockIdx.x * PK, r0 = blockIdx.y * PR, b = blockIdx.z;
-  const int nps = p.planes * p.S;
-  for (int h = 0; h < 2; ++h) {
-    int rl, kq;
-    if (p.r_major) {
-      rl = threadIdx.x % PR;
-      kq = threadIdx.x / PR + 8 * h;
-    } else {
-      kq = threadIdx.x % (PK / 4);
-      rl = threadIdx.x / (PK / 4) + 16 * h;
+  if (p.parts) {
+    if (threadIdx.x < PR) {
+      const int r = r0 + threadIdx.x;
+      int32_t e = 0;
+      if (r < p.R) {
+        double mx = 0.0;
+        for (int j = 0; j < p.nparts; ++j) mx = fmax(mx, p.parts[((int64_t)j * p.B + b) * p.R + r]);
+        e = scale_exp(mx, p.bias);
+        if (blockIdx.x == 0) p.exps[(int64_t)b * p.R + r] = e;
+      }
+      s_exp[threadIdx.x] = e;
     }
-    const int r = r0 + rl;
-    uint32_t acc[3][MAXS][WPQ];
+    __syncthreads();
+  }
+  int rl[2], kq[2];
+  double sc[2];
+  Cplx<double> x[2][4];
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl)
-#pragma unroll
-      for (int s = 0; s < MAXS; ++s)
-#pragma unroll
-        for (int w = 0; w < WPQ; ++w) acc[pl][s][w] = 0;
-    const double sc = (r < p.R && p.exps) ? ldexp(1.0, -p.exps[(int64_t)b * p.R + r]) : 1.0;
+  for (int h = 0; h < 2; ++h) {
+    if (p.r_major) {
+      rl[h] = threadIdx.x % PR;
+      kq[h] = threadIdx.x / PR + (NT / PR) * h;
+    } else {
+      kq[h] = threadIdx.x % (PK / 4);
+      rl[h] = threadIdx.x / (PK / 4) + (NT / (PK / 4)) * h;
+    }
+    const int r = r0 + rl[h];
+    sc[h] = 1.0;
+    if (r < p.R) {
+      if (p.parts) sc[h] = ldexp(1.0, -s_exp[rl[h]]);
+      else if (p.exps) sc[h] = ldexp(1.0, -p.exps[(int64_t)b * p.R + r]);
+    }
     const int64_t rbase = r < p.R ? p.bX[b] + p.rowX[r] : 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int k = k0 + 4 * kq + e;
-      double v[3] = {0.0, 0.0, 0.0};
+      const int k = k0 + 4 * kq[h] + e;
+      x[h][e] = Cplx<double>{0.0, 0.0};
       if (r < p.R && k < p.K) {
         const int64_t off = rbase + p.kX[k];
-        const Cplx<double> x = p.cplx ? load_elem<double, true>(p.X, off) : load_elem<double, false>(p.X, off);
-        v[0] = x.re * sc;
-        v[1] = x.im * sc;
-        v[2] = (x.re + x.im) * sc;
+        x[h][e] = p.cplx ? load_elem<double, true>(p.X, off) : load_elem<double, false>(p.X, off);
       }
-      const int word = (e * ES) / 4, shift = ((e * ES) % 4) * 8;
+    }
+  }
+  constexpr int CPR = PK * ES / 16;  // 16-byte chunks per tile row
+  const int total = p.S * PR * CPR;
+  const int64_t slot = (int64_t)p.Kp * ES, rowbytes = slot * p.S;
+  for (int pl = 0; pl < p.planes; ++pl) {
+    if (pl > 0) __syncthreads();  // the previous plane's tile has been written out
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) {
-        if (pl >= p.planes) break;
+    for (int h = 0; h < 2; ++h) {
+      uint32_t acc[MAXS][WPQ];
+#pragma unroll
+      for (int s = 0; s < MAXS; ++s)
+#pragma unroll
+        for (int w = 0; w < WPQ; ++w) acc[s][w] = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double v = (pl == 0 ? x[h][e].re : pl == 1 ? x[h][e].im : x[h][e].re + x[h][e].im) * sc[h];
+        const int word = (e * ES) / 4, shift = ((e * ES) % 4) * 8;
         uint32_t tmp[MAXS];
 #pragma unroll
         for (int s = 0; s < MAXS; ++s) tmp[s] = 0;
-        slice_value<MODE>(v[pl], p.S, tmp, shift);
+        slice_value<MODE>(v, p.S, tmp, shift);
 #pragma unroll
-        for (int s = 0; s < MAXS; ++s) {
+        for (int s = 0; s < MAXS; ++s)
 #pragma unroll
           for (int w = 0; w < WPQ; ++w)
-            if (w == word) acc[pl][s][w] |= tmp[s];
-        }
+            if (w == word) acc[s][w] |= tmp[s];
       }
-    }
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-      if (pl >= p.planes) break;
 #pragma unroll
       for (int s = 0; s < MAXS; ++s) {
         if (s >= p.S) break;
 #pragma unroll
-        for (int w = 0; w < WPQ; ++w) tile[pl * p.S + s][rl][kq * WPQ + w] = acc[pl][s][w];
+        for (int w = 0; w < WPQ; ++w) tile[s][rl[h]][kq[h] * WPQ + w] = acc[s][w];
       }
     }
-  }
-  __syncthreads();
-  // write out 16-byte chunks of K-contiguous rows
-  constexpr int CPR = PK * ES / 16;  // chunks per row
-  const int total = nps * PR * CPR;
-  const int64_t slot = (int64_t)p.Kp * ES, rowbytes = slot * p.S;
-  for (int c = threadIdx.x; c < total; c += NT) {
-    const int ch = c % CPR;
-    const int rl = (c / CPR) % PR;
-    const int ps = c / (CPR * PR);
-    const int pl = ps / p.S, sl = ps - pl * p.S;
-    const int pos = p.rev ? p.S - 1 - sl : sl;
-    const uint32_t* src = &tile[ps][rl][ch * 4];
-    const uint4 val = make_uint4(src[0], src[1], src[2], src[3]);
-    uint8_t* out = (uint8_t*)p.Q + (((int64_t)pl * p.B + b) * p.Rp + r0 + rl) * rowbytes + pos * slot +
-                   (int64_t)k0 * ES + ch * 16;
-    *reinterpret_cast<uint4*>(out) = val;
+    __syncthreads();
+    for (int c = threadIdx.x; c < total; c += NT) {
+      const int ch = c % CPR;
+      const int rr = (c / CPR) % PR;
+      const int sl = c / (CPR * PR);
+      const int pos = p.rev ? p.S - 1 - sl : sl;
+      const uint32_t* src = &tile[sl][rr][ch * 4];
+      const uint4 val = make_uint4(src[0], src[1], src[2], src[3]);
+      uint8_t* out = (uint8_t*)p.Q + (((int64_t)pl * p.B + b) * p.Rp + r0 + rr) * rowbytes + pos * slot +
+                     (int64_t)k0 * ES + ch * 16;
+      *reinterpret_cast<uint4*>(out) = val;
+    }
   }
 }
 
@@ -479,8 +534,10 @@ std::vector<Product> products_for(bool ca, bool cb) {
 }
 
 struct WsLayout {
-  size_t expA, expB, QA, QB, G32, total;
+  size_t expA, expB, partsA, partsB, QA, QB, G32, total;
 };
+
+int64_t scale_parts(const Plan& p) { return (p.d.K + SCALE_KCHUNK - 1) / SCALE_KCHUNK; }
 
 WsLayout ws_layout(const Plan& p) {
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
@@ -491,6 +548,10 @@ WsLayout ws_layout(const Plan& p) {
   off += al((size_t)d.B * p.Mp * 4);
   w.expB = off;
   off += al((size_t)d.B * p.Np * 4);
+  w.partsA = off;
+  off += al((size_t)scale_parts(p) * d.B * d.M * 8);
+  w.partsB = off;
+  off += al((size_t)scale_parts(p) * d.B * d.N * 8);
   const size_t rowbytes = (size_t)p.Kp * p.esize * p.slices;  // S slices per packed row
   w.QA = off;
   off += al((size_t)p.planes_a * d.B * p.Mp * rowbytes);
@@ -835,20 +896,34 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
   const bool scaled = p.split_mode != MODE_BF16;
   const int bias = p.split_mode == MODE_I8 ? 2 : 1;
   hipStream_t s = h.stream;
+  // row-contiguous operands: partial maxima over K pieces (reduced by the
+  // pack kernel); otherwise one wavefront per row
+  const int32_t nparts = (int32_t)scale_parts(p);
+  double* partsA = (double*)(base + w.partsA);
+  double* partsB = (double*)(base + w.partsB);
   if (scaled) {
-    auto launch_scale = [&](const ScaleArgs& sa) {
-      const int64_t rows = (int64_t)sa.R * sa.B;
-      hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(NT), 0, s, sa);
+    auto launch_scale = [&](const ScaleArgs& sa, bool r_major) {
+      if (r_major) {
+        dim3 grid((unsigned)((sa.R + SCALE_ROWS - 1) / SCALE_ROWS), (unsigned)nparts, (unsigned)sa.B);
+        hipLaunchKernelGGL(scale_part_kernel, grid, dim3(NT), 0, s, sa);
+      } else {
+        const int64_t rows = (int64_t)sa.R * sa.B;
+        hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(NT), 0, s, sa);
+      }
     };
-    launch_scale(ScaleArgs{A, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), expA, (int32_t)d.M, (int32_t)d.K,
-                           (int32_t)d.B, ca, bias});
-    launch_scale(ScaleArgs{B, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), expB, (int32_t)d.N, (int32_t)d.K,
-                           (int32_t)d.B, cb, bias});
+    launch_scale(ScaleArgs{A, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), expA, partsA, (int32_t)d.M,
+                           (int32_t)d.K, (int32_t)d.B, ca, bias, SCALE_KCHUNK},
+                 !p.a_kmajor);
+    launch_scale(ScaleArgs{B, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), expB, partsB, (int32_t)d.N,
+                           (int32_t)d.K, (int32_t)d.B, cb, bias, SCALE_KCHUNK},
+                 !p.b_kmajor);
   }
-  auto launch_pack = [&](const void* X, int rowt, int kt, int bt, int32_t* exps, void* Q, int64_t R, int64_t Rp,
-                         bool cplx, bool r_major, int planes, bool rev) {
-    PackArgs pa{X, p.tab(rowt), p.tab(kt), p.tab(bt), scaled ? exps : nullptr, Q, (int32_t)R, (int32_t)d.K,
-                (int32_t)d.B, (int32_t)Rp, (int32_t)p.Kp, cplx, r_major, planes, p.slices, p.split_mode, rev};
+  auto launch_pack = [&](const void* X, int rowt, int kt, int bt, int32_t* exps, double* parts, void* Q, int64_t R,
+                         int64_t Rp, bool cplx, bool r_major, int planes, bool rev) {
+    PackArgs pa{X, p.tab(rowt), p.tab(kt), p.tab(bt), scaled ? exps : nullptr, scaled && r_major ? parts : nullptr,
+                Q, (int32_t)R, (int32_t)d.K, (int32_t)d.B, (int32_t)Rp, (int32_t)p.Kp, cplx, r_major, planes,
+                p.slices, p.split_mode, rev, nparts, bias};
+    const int PK = pack_k(p.split_mode), PR = pack_r(p.split_mode);
     dim3 grid((unsigned)(p.Kp / PK), (unsigned)(Rp / PR), (unsigned)d.B);
     switch (p.split_mode) {
       case MODE_I8: hipLaunchKernelGGL(pack_kernel<MODE_I8>, grid, dim3(NT), 0, s, pa); break;
@@ -857,8 +932,8 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
     }
   };
   // A rows are M, its "row-major" gather direction is M when A is not K-major
-  launch_pack(A, TAB_ROWA, TAB_KA, TAB_BA, expA, base + w.QA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a, false);
-  launch_pack(B, TAB_COLB, TAB_KB, TAB_BB, expB, base + w.QB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b, true);
+  launch_pack(A, TAB_ROWA, TAB_KA, TAB_BA, expA, partsA, base + w.QA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a, false);
+  launch_pack(B, TAB_COLB, TAB_KB, TAB_BB, expB, partsB, base + w.QB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b, true);
 
   const int64_t slot = p.Kp * p.esize, rowbytes = slot * p.slices;
   SlicedArgs ga;
