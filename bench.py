@@ -395,6 +395,62 @@ def bench_simple_update(dev, chis, threads: int):
     return res
 
 
+def bench_memory_bound(dev, steps: int):
+    """hadamard and unary_einsum at HBM-bound sizes (ComplexF64, 2^26-element operands): ms per call,
+    algorithmic GB/s (every operand element read once, every output element written once) and the
+    fraction of the 8 TB/s HBM peak."""
+    import numpy as np
+    import torch
+
+    import muscle
+    I = muscle.Index
+    res = {}
+    es = 16
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+
+    def rnd(shape):
+        return torch.randn(shape, dtype=torch.complex128, device=dev, generator=g)
+
+    def colmajor(shape):
+        return torch.empty(tuple(reversed(shape)), dtype=torch.complex128, device=dev).permute(
+            *reversed(range(len(shape))))
+
+    def tens(x, labels):
+        return muscle.Tensor(x, [I(l) for l in labels])
+
+    ni, nj, nk = 256, 512, 512
+    x = tens(rnd((nk, nj, ni)).permute(2, 1, 0), "ijk")  # column-major (i fastest), 2^26 elements
+    n = ni * nj * nk
+    cases = []
+    c = tens(colmajor((ni, nj, nk)), "ijk")
+    bj = tens(rnd((nj,)), "j")
+    cases.append(("hadamard_broadcast", lambda: muscle.hadamard_(c, x, bj), n + n + nj, "c[i,j,k] = a[i,j,k] b[j]"))
+    bp = tens(rnd((nj, ni, nk)).permute(2, 1, 0), "kij")  # b[k,i,j], column-major in its own order
+    cases.append(("hadamard_permuted", lambda: muscle.hadamard_(c, x, bp), 3 * n, "c[i,j,k] = a[i,j,k] b[k,i,j]"))
+    yk = tens(colmajor((nk,)), "k")
+    cases.append(("unary_sum_fast_modes", lambda: muscle.unary_einsum_(yk, x), n + nk, "y[k] = sum_ij x[i,j,k]"))
+    yij = tens(colmajor((ni, nj)), "ij")
+    cases.append(("unary_sum_slow_mode", lambda: muscle.unary_einsum_(yij, x), n + ni * nj,
+                  "y[i,j] = sum_k x[i,j,k]"))
+    ykji = tens(colmajor((nk, nj, ni)), "kji")
+    cases.append(("unary_permute", lambda: muscle.unary_einsum_(ykji, x), 2 * n, "y[k,j,i] = x[i,j,k]"))
+    for name, fn, elems, what in cases:
+        ms, _ = timed(fn, steps, 2)
+        gbs = elems * es / (ms * 1e-3) / 1e9
+        res[name] = {"op": what, "ms": round(ms, 4), "gb_per_s": round(gbs, 1),
+                     "frac_hbm_peak": round(gbs / (HBM_TBS * 1e3), 4)}
+    # a trace: y[k] = sum_i x[i,i,k] over a 4096 x 4096 x 4 tensor (touches 16 K elements)
+    xt = tens(rnd((4, 4096, 4096)).permute(2, 1, 0), "iik")
+    yt = tens(colmajor((4,)), "k")
+    ms, _ = timed(lambda: muscle.unary_einsum_(yt, xt), steps, 2)
+    res["unary_trace"] = {"op": "y[k] = sum_i x[i,i,k] (4096 x 4096 x 4)", "ms": round(ms, 4),
+                          "note": "latency-bound: 16 K diagonal elements"}
+    del x, c, bp, yk, yij, ykji, xt
+    torch.cuda.empty_cache()
+    return res
+
+
 def measured_traffic(precision: str):
     """HBM bytes per launch of the headline kernel, from the committed rocprofv3
     FETCH_SIZE + WRITE_SIZE summary (profiles/traffic_config2_c64.txt), or None."""
@@ -417,7 +473,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only", default="all",
-                    help="comma list of: ladder_c2,c2_c32,c2h_c64,dims2,c1_c64,ladder_c1,batched,served,simple_update "
+                    help="comma list of: ladder_c2,c2_c32,c2h_c64,dims2,c1_c64,ladder_c1,batched,served,memory_bound,"
+                         "simple_update "
                          "(the headline always runs)")
     ap.add_argument("--precision", default="native", help="arithmetic of the headline workload")
     ap.add_argument("--target", type=float, default=1e-10)
@@ -599,6 +656,13 @@ def main() -> int:
                 log(f"served {dist} failed: {e!r}")
             torch.cuda.empty_cache()
 
+    if want("memory_bound"):
+        try:
+            extra["memory_bound_c64"] = bench_memory_bound(dev, max(5, args.steps))
+            log(f"memory-bound: {extra['memory_bound_c64']}")
+        except Exception as e:  # reported in the JSON, not hidden
+            extra["memory_bound_c64"] = {"error": repr(e)}
+            log(f"memory-bound failed: {e!r}")
     if want("simple_update"):
         try:
             extra["simple_update_c64"] = bench_simple_update(dev, [64, 256, 1024], 0 if args.no_cpu_baseline

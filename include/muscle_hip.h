@@ -98,7 +98,8 @@ typedef struct {
 /* What the library did for one contraction (optional out-parameter). */
 typedef struct {
   int32_t kernel;        /* 0 none, 1 MFMA tile GEMM, 2 reduction, 3 split-precision, 4 elementwise,
-                            5 split-precision with the int8 slice products as hipBLASLt GEMMs */
+                            5 split-precision with the int8 slice products as hipBLASLt GEMMs,
+                            6 tiled permutation (unary_einsum with no summed modes, transposed through LDS) */
   int32_t precision;     /* muscle_hip_precision_t actually used */
   int32_t slices;        /* Ozaki slices per operand (0 otherwise) */
   int32_t swapped;       /* 1 if A and B were exchanged to make C's fastest mode an N mode */

@@ -135,8 +135,58 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   const int64_t lim32 = int64_t(1) << 32;
   if (d.M >= lim32 || d.N >= lim32 || d.K >= lim32 || d.B >= lim32)
     fail(MUSCLE_HIP_UNSUPPORTED, "an index class with 2^32 or more elements");
+  // A unary permutation (no summed modes) whose input and output run
+  // contiguous along different modes is a tiled transpose: the tile rows are
+  // the output's fastest dims (at least 32 elements), the tile columns the
+  // input's fastest remaining dims, whose input strides move to the B slot.
+  ContractionDesc pd;
+  bool permute = false;
+  if (!d.has_b && d.k.empty() && d.M >= (int64_t(1) << 14) && d.m.size() >= 2) {
+    std::vector<int> byx(d.m.size());
+    for (size_t i = 0; i < byx.size(); ++i) byx[i] = (int)i;
+    auto xs = [&](int i) { return d.m[i].s[0] < 0 ? -d.m[i].s[0] : d.m[i].s[0]; };
+    std::stable_sort(byx.begin(), byx.end(), [&](int u, int v) { return xs(u) < xs(v); });
+    std::vector<char> row(d.m.size(), 0), col(d.m.size(), 0);
+    int64_t pr = 1;
+    for (size_t i = 0; i < d.m.size() && pr < 32; ++i) {
+      row[i] = 1;
+      pr *= d.m[i].extent;
+    }
+    int64_t pc = 1;
+    for (int j : byx) {
+      if (pc >= 32) break;
+      if (row[j]) continue;
+      col[j] = 1;
+      pc *= d.m[j].extent;
+    }
+    int first_col = -1;
+    for (int j : byx)
+      if (col[j]) {
+        first_col = j;
+        break;
+      }
+    if (byx[0] != 0 && first_col >= 0 && xs(first_col) < xs(0)) {
+      permute = true;
+      pd = d;
+      pd.m.clear();
+      pd.n.clear();
+      for (size_t i = 0; i < d.m.size(); ++i)
+        if (!col[i]) pd.m.push_back(d.m[i]);
+      for (int j : byx)
+        if (col[j]) {
+          Dim x = d.m[j];
+          x.s[1] = x.s[0];
+          x.s[0] = 0;
+          pd.n.push_back(x);
+        }
+      pd.M = pd.N = 1;
+      for (auto& x : pd.m) pd.M *= x.extent;
+      for (auto& x : pd.n) pd.N *= x.extent;
+    }
+  }
+  const ContractionDesc& td = permute ? pd : d;
   // two-level tables: (class dims, operand slot) per TabId
-  const std::vector<Dim>* cls[TAB_COUNT] = {&d.m, &d.k, &d.k, &d.n, &d.m, &d.n, &d.batch, &d.batch, &d.batch};
+  const std::vector<Dim>* cls[TAB_COUNT] = {&td.m, &td.k, &td.k, &td.n, &td.m, &td.n, &td.batch, &td.batch, &td.batch};
   const int slot[TAB_COUNT] = {0, 0, 1, 1, 2, 2, 0, 1, 2};
   for (int t = 0; t < TAB_COUNT; ++t) {
     std::vector<Dim> lo, hi;
@@ -149,6 +199,11 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
     p->tab_hi[t] = at[1];
     p->tab_P[t] = (uint32_t)P;
     p->tab_single[t] = hi.empty() ? 1u : 0u;
+  }
+  if (permute) {
+    p->d = pd;
+    p->kind = Plan::PERMUTE;
+    return p;
   }
 
   auto min_stride = [](const std::vector<Dim>& v, int w) {
@@ -195,6 +250,15 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   const int64_t outputs = d.M * d.N * d.B;
   int gshift = 0;
   while (gshift < 6 && (int64_t(8) << gshift) < d.K) ++gshift;
+  // a unary sum over modes the input holds at larger strides than the
+  // outputs' fastest dim: one lane per output, so every load instruction
+  // reads neighbouring elements instead of lanes striding over k
+  if (!d.has_b && !d.m.empty() && !d.k.empty() && outputs >= 64) {
+    int64_t kmin = INT64_MAX;
+    for (auto& x : d.k) kmin = std::min<int64_t>(kmin, x.s[0] < 0 ? -x.s[0] : x.s[0]);
+    const int64_t mfast = d.m[0].s[0] < 0 ? -d.m[0].s[0] : d.m[0].s[0];
+    if (kmin > mfast) gshift = 0;
+  }
   int64_t threads = outputs << gshift;
   int64_t nsplit = 1;
   const int64_t target = 256 * 1024;
@@ -314,7 +378,11 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
   if (!info) return;
   std::memset(info, 0, sizeof *info);
   const ContractionDesc& d = p.d;
-  info->kernel = p.kind == Plan::GEMM ? 1 : p.kind == Plan::REDUCE ? 2 : p.kind == Plan::SPLIT ? (p.split_lib ? 5 : 3) : 0;
+  info->kernel = p.kind == Plan::GEMM      ? 1
+                 : p.kind == Plan::REDUCE  ? 2
+                 : p.kind == Plan::SPLIT   ? (p.split_lib ? 5 : 3)
+                 : p.kind == Plan::PERMUTE ? 6
+                                           : 0;
   info->precision = p.fp32_math ? MUSCLE_HIP_PRECISION_FP32
                                  : (p.kind == Plan::SPLIT ? p.precision : MUSCLE_HIP_PRECISION_NATIVE);
   info->slices = p.slices;
@@ -367,6 +435,13 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   const void* B = kb ? promote_operand(h, *kb, dbl, cursor, dtb) : nullptr;
   if (p.kind == Plan::SPLIT) {
     run_split(h, p, c.data, A, B, is_complex(dta), is_complex(dtb), (char*)h.ws + split_off);
+    check_launch();
+    return;
+  }
+  if (p.kind == Plan::PERMUTE) {
+    launch_permute((int)csize, PermuteArgs{A, c.data, p.tab(TAB_ROWA), p.tab(TAB_ROWC), p.tab(TAB_COLB),
+                                           p.tab(TAB_COLC), d.M, d.N},
+                   h.stream);
     check_launch();
     return;
   }

@@ -155,7 +155,43 @@ __global__ __launch_bounds__(NT) void pack_operand_kernel(PackOpArgs p) {
     if (r < p.R && k < p.K) dst[(int64_t)r * p.K + k] = tile[rl][kl];
   }
 }
+constexpr int PMT = 32;  // permutation tile: 32 x 32 elements
+
+template <typename E>
+__global__ __launch_bounds__(NT) void permute_tile_kernel(PermuteArgs p) {
+  __shared__ E tile[PMT][PMT + 1];
+  const int64_t tiles_m = (p.M + PMT - 1) / PMT;
+  const int64_t t = blockIdx.x;
+  const int64_t m0 = (t % tiles_m) * PMT, n0 = (t / tiles_m) * PMT;
+  const E* X = reinterpret_cast<const E*>(p.X);
+  E* Y = reinterpret_cast<E*>(p.Y);
+  // read along n (X's contiguous direction)
+  for (int e = threadIdx.x; e < PMT * PMT; e += NT) {
+    const int nl = e % PMT, ml = e / PMT;
+    const int64_t m = m0 + ml, n = n0 + nl;
+    if (m < p.M && n < p.N) tile[ml][nl] = X[p.rowX[(uint32_t)m] + p.colX[(uint32_t)n]];
+  }
+  __syncthreads();
+  // write along m (Y's contiguous direction)
+  for (int e = threadIdx.x; e < PMT * PMT; e += NT) {
+    const int ml = e % PMT, nl = e / PMT;
+    const int64_t m = m0 + ml, n = n0 + nl;
+    if (m < p.M && n < p.N) Y[p.rowY[(uint32_t)m] + p.colY[(uint32_t)n]] = tile[ml][nl];
+  }
+}
+
 }  // namespace
+
+void launch_permute(int elem_bytes, const PermuteArgs& a, hipStream_t s) {
+  const int64_t tiles = ((a.M + PMT - 1) / PMT) * ((a.N + PMT - 1) / PMT);
+  if (tiles <= 0) return;
+  const dim3 g((unsigned)tiles), b(NT);
+  switch (elem_bytes) {
+    case 4: hipLaunchKernelGGL(permute_tile_kernel<uint32_t>, g, b, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(permute_tile_kernel<uint2>, g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL(permute_tile_kernel<uint4>, g, b, 0, s, a); break;
+  }
+}
 
 void launch_pack_operand(int elem_bytes, const PackOpArgs& a, hipStream_t s) {
   const int64_t tiles = ((int64_t)(a.K + PTK - 1) / PTK) * ((a.R + PTR - 1) / PTR);

@@ -20,7 +20,7 @@ enum TabId { TAB_ROWA = 0, TAB_KA, TAB_KB, TAB_COLB, TAB_ROWC, TAB_COLC, TAB_BA,
 
 // A lowered contraction with its offset tables resident on the device.
 struct Plan {
-  enum Kind { EMPTY = 0, GEMM = 1, REDUCE = 2, SPLIT = 3 };
+  enum Kind { EMPTY = 0, GEMM = 1, REDUCE = 2, SPLIT = 3, PERMUTE = 4 };
   int kind = EMPTY;
   int precision = MUSCLE_HIP_PRECISION_NATIVE;
   ContractionDesc d;

@@ -120,6 +120,18 @@ struct PackOpArgs {
 };
 void launch_pack_operand(int elem_bytes, const PackOpArgs& a, hipStream_t s);
 
+// ---- tiled permutation (unary_einsum without summed modes) -------------------
+// Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]]: m runs along Y's fastest dims,
+// n along X's fastest remaining dims; 32 x 32 tiles transposed through LDS so
+// that both the reads and the writes are contiguous runs.
+struct PermuteArgs {
+  const void* X;
+  void* Y;
+  OffTab rowX, rowY, colX, colY;
+  int64_t M, N;
+};
+void launch_permute(int elem_bytes, const PermuteArgs& a, hipStream_t s);
+
 // dst (k x n, ld ldd) = upper trapezoid of the first k rows of src (m x n, ld m)
 void launch_upper_trapezoid(int elem_bytes, const void* src, void* dst, int64_t m, int64_t k, int64_t n, int64_t ldd,
                             hipStream_t s);

@@ -132,6 +132,56 @@ def test_unary_parity(case, T):
     assert rel_err(Y, ref) <= tol
 
 
+DIMS2 = [f"x{i}" for i in range(16)]
+PCASES = [
+    (["i", "j", "k"], (64, 40, 48), ["k", "j", "i"]),
+    (["i", "j", "k"], (64, 40, 48), ["j", "k", "i"]),
+    (["i", "j", "k"], (33, 1, 997), ["k", "j", "i"]),  # ragged, an extent-1 mode
+    (["i", "j", "i", "k"], (40, 50, 40, 30), ["k", "i", "j"]),  # diagonal and permutation
+    (DIMS2, (2,) * 16, list(np.random.default_rng(3).permutation(DIMS2))),  # dims 2, random order
+]
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.float64, np.complex64, np.float32])
+@pytest.mark.parametrize("case", PCASES, ids=lambda c: "".join(c[0])[:12] + "->" + "".join(c[2])[:12])
+def test_unary_permutation_tiled(case, T):
+    """Pure permutations (no summed modes) of >= 2^14 elements whose input and output
+    run contiguous along different modes go to the tiled transpose (kernel 6): a copy,
+    so the result is bit-exact."""
+    ix, sx, iy = case
+    X = randn(np.random.default_rng(19), sx, T)
+    ref = bb.unary_einsum_base(iy, X, ix)
+    y = muscle.unary_einsum(dev_tensor(X, ix), out=iy)
+    assert muscle.last_info()["kernel"] == 6
+    assert np.array_equal(host(y), ref)
+
+
+def test_unary_permutation_into_strided_view():
+    X = randn(np.random.default_rng(20), (48, 36, 20), np.complex128)
+    big = torch.zeros((36, 3, 48, 20), dtype=torch.complex128, device="cuda")
+    yv = muscle.Tensor(big[:, 1, :, :], ["j", "i", "k"])  # a strided (non-contiguous) output view
+    muscle.unary_einsum_(yv, dev_tensor(X, ["i", "j", "k"]))
+    assert muscle.last_info()["kernel"] == 6  # y runs contiguous along k, x along i
+    got = big.cpu().numpy()
+    assert np.array_equal(got[:, 1], np.transpose(X, (1, 0, 2)))
+    assert not got[:, 0].any() and not got[:, 2].any()
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.float32])
+@pytest.mark.parametrize("case", [(["i", "j", "k"], (256, 64, 128), ["i", "j"]),
+                                  (["i", "j", "k"], (128, 64, 33), ["j", "i"]),
+                                  (["i", "j", "k"], (4, 8, 4096), ["j", "i"])],
+                         ids=lambda c: "".join(c[0]) + "->" + "".join(c[2]))
+def test_unary_sum_over_slow_modes(case, T):
+    """Sums over modes the input holds at large strides (one lane per output)."""
+    ix, sx, iy = case
+    X = randn(np.random.default_rng(21), sx, T)
+    ref = bb.unary_einsum_base(iy, X.astype(np.complex128), ix)
+    y = muscle.unary_einsum(dev_tensor(X, ix), out=iy)
+    tol = 1e-13 if T in (np.float64, np.complex128) else 1e-5
+    assert rel_err(host(y), ref) <= tol
+
+
 def test_unary_golden():
     g = np.load(os.path.join(GOLDEN, "unary_c64.npz"))
     y = muscle.unary_einsum(dev_tensor(g["x"], ["i", "j", "i", "k"]), out=["k", "j"])
