@@ -517,6 +517,10 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   r.gshift = p.gshift;
   r.nsplit = p.nsplit;
   r.small_index = r.outputs < (int64_t(1) << 32) ? 1 : 0;
+  if (d.k.size() == 1) {
+    r.kstepA = d.k[0].s[0];
+    r.kstepB = d.k[0].s[1];
+  }
   if (r.small_index) {
     r.divN = FastDiv((uint32_t)d.N);
     r.divM = FastDiv((uint32_t)d.M);
@@ -540,6 +544,70 @@ Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_h
   upload_split_tables(*np, h.stream);
   return h.insert_plan(key, std::move(np));
 }
+
+// ---- elementwise planning
+
+// When b (with 2^14 or more elements) runs contiguous along other modes than
+// c, the product goes through 32 x 32 tiles: the tile rows are c's fastest
+// dims, the columns b's fastest remaining dims (reordered into w).
+void plan_hadamard_tiles(const ElementwiseDesc& e, EwArgs& w) {
+  const std::vector<Dim>& d = e.dims;
+  if (e.total < (int64_t(1) << 16) || d.size() < 2) return;
+  auto sb = [&](size_t i) { return d[i].s[1] < 0 ? -d[i].s[1] : d[i].s[1]; };
+  int64_t nb = 1;
+  for (size_t i = 0; i < d.size(); ++i)
+    if (d[i].s[1] != 0) nb *= d[i].extent;
+  if (nb < (int64_t(1) << 14) || sb(0) == 0) return;  // small b, or b broadcast along c's fastest dim
+  std::vector<size_t> byb;
+  for (size_t i = 0; i < d.size(); ++i)
+    if (d[i].s[1] != 0) byb.push_back(i);
+  std::stable_sort(byb.begin(), byb.end(), [&](size_t u, size_t v) { return sb(u) < sb(v); });
+  if (byb[0] == 0) return;  // b and c both run along c's fastest dim
+  std::vector<char> row(d.size(), 0), col(d.size(), 0);
+  int64_t pr = 1;
+  for (size_t i = 0; i < d.size() && pr < 32; ++i) {
+    row[i] = 1;
+    pr *= d[i].extent;
+  }
+  int64_t pc = 1;
+  size_t first = d.size();
+  for (size_t j : byb) {
+    if (pc >= 32) break;
+    if (row[j]) continue;
+    if (first == d.size()) first = j;
+    col[j] = 1;
+    pc *= d[j].extent;
+  }
+  if (first == d.size() || sb(first) >= sb(0)) return;
+  std::vector<size_t> order;
+  for (size_t i = 0; i < d.size(); ++i)
+    if (row[i]) order.push_back(i);
+  const int32_t Rr = (int32_t)order.size();
+  for (size_t j : byb)
+    if (col[j]) order.push_back(j);
+  const int32_t Rc = (int32_t)order.size() - Rr;
+  for (size_t i = 0; i < d.size(); ++i)
+    if (!row[i] && !col[i]) order.push_back(i);
+  int64_t nrows = 1, ncols = 1, nouter = 1;
+  for (size_t k = 0; k < order.size(); ++k)
+    (k < (size_t)Rr ? nrows : k < (size_t)(Rr + Rc) ? ncols : nouter) *= d[order[k]].extent;
+  if ((double)((nrows + 31) / 32) * (double)((ncols + 31) / 32) * (double)nouter >= 2147483648.0) return;
+  w.tiled = 1;
+  w.Rr = Rr;
+  w.Rc = Rc;
+  w.nrows = nrows;
+  w.ncols = ncols;
+  w.nouter = nouter;
+  for (size_t k = 0; k < order.size(); ++k) {
+    const Dim& x = d[order[k]];
+    w.ext[k] = x.extent;
+    w.sA[k] = x.s[0];
+    w.sB[k] = x.s[1];
+    w.sC[k] = x.s[2];
+    w.div[k] = FastDiv(w.small_index ? (uint32_t)x.extent : 1u);
+  }
+}
+
 
 }  // namespace mh
 
@@ -746,6 +814,7 @@ int muscle_hip_hadamard(muscle_hip_handle_t handle, const muscle_hip_tensor_t* c
       w.R = 1;
       w.ext[0] = 1;
     }
+    plan_hadamard_tiles(e, w);
     std::lock_guard<std::mutex> lk(h->mu);
     launch_hadamard(a->dtype, b->dtype, w, h->stream);
     check_launch();

@@ -67,8 +67,87 @@ __global__ __launch_bounds__(NT) void hadamard_kernel(EwArgs p) {
   }
 }
 
+// Offsets in A, B and C of index i over dims [d0, d1) (first fastest).
+__device__ inline void ew_offsets(const EwArgs& p, int d0, int d1, int64_t i, int64_t& oa, int64_t& ob,
+                                  int64_t& oc) {
+  oa = ob = oc = 0;
+  for (int d = d0; d < d1; ++d) {
+    const int64_t q = i / p.ext[d];
+    const int64_t x = i - q * p.ext[d];
+    oa += x * p.sA[d];
+    ob += x * p.sB[d];
+    oc += x * p.sC[d];
+    i = q;
+  }
+}
+
+constexpr int HT = 32;  // hadamard tile: 32 x 32 elements
+
+// b is read along the tile columns (its contiguous direction) into LDS, then
+// a and c along the tile rows (theirs).
+template <typename TA, bool CA, typename TB, bool CB>
+__global__ __launch_bounds__(NT) void hadamard_tile_kernel(EwArgs p) {
+  typedef decltype(TA(0) * TB(0)) TP;
+  __shared__ Cplx<TB> tb[HT][HT + 1];
+  __shared__ int64_t roff[3][HT], coff[3][HT];
+  const int64_t tiles_m = (p.nrows + HT - 1) / HT, tiles_n = (p.ncols + HT - 1) / HT;
+  int64_t t = blockIdx.x;
+  const int64_t m0 = (t % tiles_m) * HT;
+  t /= tiles_m;
+  const int64_t n0 = (t % tiles_n) * HT;
+  const int64_t q = t / tiles_n;
+  if (threadIdx.x < 2 * HT) {
+    const bool is_row = threadIdx.x < HT;
+    const int l = threadIdx.x % HT;
+    const int64_t i = is_row ? m0 + l : n0 + l;
+    int64_t oa = 0, ob = 0, oc = 0;
+    if (i < (is_row ? p.nrows : p.ncols)) {
+      if (is_row)
+        ew_offsets(p, 0, p.Rr, i, oa, ob, oc);
+      else
+        ew_offsets(p, p.Rr, p.Rr + p.Rc, i, oa, ob, oc);
+    }
+    int64_t(*dst)[HT] = is_row ? roff : coff;
+    dst[0][l] = oa;
+    dst[1][l] = ob;
+    dst[2][l] = oc;
+  }
+  int64_t qa, qb, qc;
+  ew_offsets(p, p.Rr + p.Rc, p.R, q, qa, qb, qc);
+  __syncthreads();
+  for (int e = threadIdx.x; e < HT * HT; e += NT) {
+    const int nl = e % HT, ml = e / HT;
+    if (m0 + ml < p.nrows && n0 + nl < p.ncols) tb[ml][nl] = load_elem<TB, CB>(p.B, qb + roff[1][ml] + coff[1][nl]);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < HT * HT; e += NT) {
+    const int ml = e % HT, nl = e / HT;
+    if (m0 + ml >= p.nrows || n0 + nl >= p.ncols) continue;
+    const Cplx<TA> a = load_elem<TA, CA>(p.A, qa + roff[0][ml] + coff[0][nl]);
+    const Cplx<TB> b = tb[ml][nl];
+    const TP ar = a.re, ai = a.im, br = b.re, bi = b.im;
+    Cplx<TA> c;
+    if constexpr (CA && CB) {
+      c.re = (TA)(ar * br - ai * bi);
+      c.im = (TA)(ar * bi + ai * br);
+    } else if constexpr (CA) {
+      c.re = (TA)(ar * br);
+      c.im = (TA)(ai * br);
+    } else {
+      c.re = (TA)(ar * br);
+      c.im = TA(0);
+    }
+    store_elem<TA, CA>(p.C, qc + roff[2][ml] + coff[2][nl], c);
+  }
+}
+
 template <typename TA, bool CA, typename TB, bool CB>
 void launch_h(const EwArgs& e, hipStream_t s) {
+  if (e.tiled) {
+    const int64_t tiles = ((e.nrows + HT - 1) / HT) * ((e.ncols + HT - 1) / HT) * e.nouter;
+    hipLaunchKernelGGL((hadamard_tile_kernel<TA, CA, TB, CB>), dim3((unsigned)tiles), dim3(NT), 0, s, e);
+    return;
+  }
   int64_t blocks = (e.total + NT - 1) / NT;
   if (blocks > 65536) blocks = 65536;  // grid-stride beyond ~16M threads
   if (blocks < 1) blocks = 1;
@@ -160,23 +239,36 @@ constexpr int PMT = 32;  // permutation tile: 32 x 32 elements
 template <typename E>
 __global__ __launch_bounds__(NT) void permute_tile_kernel(PermuteArgs p) {
   __shared__ E tile[PMT][PMT + 1];
+  __shared__ int64_t off[4][PMT];  // rowX, colX, rowY, colY of the tile
   const int64_t tiles_m = (p.M + PMT - 1) / PMT;
   const int64_t t = blockIdx.x;
   const int64_t m0 = (t % tiles_m) * PMT, n0 = (t / tiles_m) * PMT;
   const E* X = reinterpret_cast<const E*>(p.X);
   E* Y = reinterpret_cast<E*>(p.Y);
+  if (threadIdx.x < 2 * PMT) {
+    const int l = threadIdx.x % PMT;
+    if (threadIdx.x < PMT) {
+      const int64_t m = m0 + l;
+      off[0][l] = m < p.M ? p.rowX[(uint32_t)m] : 0;
+      off[2][l] = m < p.M ? p.rowY[(uint32_t)m] : 0;
+    } else {
+      const int64_t n = n0 + l;
+      off[1][l] = n < p.N ? p.colX[(uint32_t)n] : 0;
+      off[3][l] = n < p.N ? p.colY[(uint32_t)n] : 0;
+    }
+  }
+  __syncthreads();
+  const int mmax = (int)min<int64_t>(PMT, p.M - m0), nmax = (int)min<int64_t>(PMT, p.N - n0);
   // read along n (X's contiguous direction)
   for (int e = threadIdx.x; e < PMT * PMT; e += NT) {
     const int nl = e % PMT, ml = e / PMT;
-    const int64_t m = m0 + ml, n = n0 + nl;
-    if (m < p.M && n < p.N) tile[ml][nl] = X[p.rowX[(uint32_t)m] + p.colX[(uint32_t)n]];
+    if (ml < mmax && nl < nmax) tile[ml][nl] = X[off[0][ml] + off[1][nl]];
   }
   __syncthreads();
   // write along m (Y's contiguous direction)
   for (int e = threadIdx.x; e < PMT * PMT; e += NT) {
     const int ml = e % PMT, nl = e / PMT;
-    const int64_t m = m0 + ml, n = n0 + nl;
-    if (m < p.M && n < p.N) Y[p.rowY[(uint32_t)m] + p.colY[(uint32_t)n]] = tile[ml][nl];
+    if (ml < mmax && nl < nmax) Y[off[2][ml] + off[3][nl]] = tile[ml][nl];
   }
 }
 

@@ -85,6 +85,9 @@ struct ReduceArgs {
   int32_t nsplit;
   FastDiv divN, divM;  // valid when outputs < 2^32
   int32_t small_index;  // 1: use FastDiv
+  // nonzero when the summed dims fuse into one: offsets k * kstep instead of
+  // the k tables
+  int64_t kstepA, kstepB;
 };
 void launch_reduce(bool dbl, bool ca, bool cb, bool has_b, const ReduceArgs& r, hipStream_t s);
 
@@ -100,6 +103,11 @@ struct EwArgs {
   FastDiv div[kMaxEwDims];
   int64_t ext[kMaxEwDims];
   int64_t sA[kMaxEwDims], sB[kMaxEwDims], sC[kMaxEwDims];
+  // tiled form (b runs contiguous along other modes than c): dims [0, Rr) are
+  // the tile rows (c's fastest), [Rr, Rr + Rc) the tile columns (b's fastest
+  // remaining), the rest the outer index
+  int32_t tiled, Rr, Rc;
+  int64_t nrows, ncols, nouter;
 };
 // dtA/dtB: muscle_hip_dtype_t of a and b; c has a's type.
 void launch_hadamard(int dtA, int dtB, const EwArgs& e, hipStream_t s);

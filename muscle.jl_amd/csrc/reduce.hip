@@ -56,10 +56,12 @@ __global__ __launch_bounds__(NT) void reduce_kernel(ReduceArgs p) {
     const int64_t k0 = (int64_t)split * p.kchunk;
     int64_t k1 = k0 + p.kchunk;
     if (k1 > p.K) k1 = p.K;
+    const bool affine = p.kstepA != 0 && (!HAS_B || p.kstepB != 0);
+#pragma unroll 4
     for (int64_t k = k0 + r; k < k1; k += G) {
-      const Cplx<T> x = load_elem<T, CA>(p.A, baseA + p.kA[k]);
+      const Cplx<T> x = load_elem<T, CA>(p.A, baseA + (affine ? k * p.kstepA : p.kA[k]));
       if constexpr (HAS_B) {
-        const Cplx<T> y = load_elem<T, CB>(p.B, baseB + p.kB[k]);
+        const Cplx<T> y = load_elem<T, CB>(p.B, baseB + (affine ? k * p.kstepB : p.kB[k]));
         if constexpr (CA && CB) {
           ar = fma(x.re, y.re, ar);
           ar = fma(-x.im, y.im, ar);

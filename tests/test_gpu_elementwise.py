@@ -132,6 +132,33 @@ def test_unary_parity(case, T):
     assert rel_err(Y, ref) <= tol
 
 
+HTCASES = [
+    (["i", "j", "k"], (64, 40, 48), ["k", "i", "j"]),
+    (["i", "j", "k"], (128, 8, 160), ["k", "i"]),  # b broadcast along j, runs along k
+    (["i", "j", "k", "l"], (33, 17, 29, 5), ["l", "k", "j", "i"]),  # ragged
+    ([f"x{i}" for i in range(16)], (2,) * 16, list(np.random.default_rng(4).permutation([f"x{i}" for i in range(16)]))),
+]
+
+
+@pytest.mark.parametrize("ta, tb", [(np.complex128, np.complex128), (np.complex128, np.float64),
+                                    (np.float32, np.float32), (np.complex64, np.complex128)])
+@pytest.mark.parametrize("case", HTCASES, ids=lambda c: "".join(c[0])[:12] + "*" + "".join(c[2])[:12])
+def test_hadamard_tiled(case, ta, tb):
+    """b (>= 2^14 elements) running contiguous along other modes than a and c: the 32 x 32
+    tile path.  Elementwise products are exact against the oracle's in the same precision."""
+    ia, sa, ib = case
+    rng = np.random.default_rng(23)
+    A = randn(rng, sa, ta)
+    sb = tuple(sa[ia.index(i)] for i in ib)
+    B = randn(rng, sb, tb)
+    ref, _ = bb.hadamard_base(A.astype(np.complex128), ia, B.astype(np.complex128), ib)
+    c = muscle.hadamard(dev_tensor(A, ia), dev_tensor(B, ib))
+    got = host(c)
+    assert got.dtype == ta
+    tol = 1e-15 if ta == np.complex128 and tb in (np.complex128, np.float64) else 1e-6
+    assert rel_err(got, ref) <= tol
+
+
 DIMS2 = [f"x{i}" for i in range(16)]
 PCASES = [
     (["i", "j", "k"], (64, 40, 48), ["k", "j", "i"]),
