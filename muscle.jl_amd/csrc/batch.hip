@@ -182,13 +182,28 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
 #undef MH_GATHER
 #undef MH_STASH
 #undef MH_COMPUTE
+  // C's offsets for this thread's rows and columns, read before any store
+  // (a table load after a store to C could not be hoisted: they may alias)
+  int64_t crow[TM][4], ccol[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wm * (BM / WM) + i * 16 + MM::row(lane, r);
+      crow[i][r] = m < M ? pr.rowC[m] : 0;
+    }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
+    ccol[j] = n < N ? offC0 + pr.colC[n] : 0;
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
       if (n >= N) continue;
-      const int64_t cn = offC0 + pr.colC[n];
+      const int64_t cn = ccol[j];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * (BM / WM) + i * 16 + MM::row(lane, r);
@@ -201,7 +216,7 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
           re = acc0[i][j][r];
           im = acc1[i][j][r];
         }
-        store_elem<T, CP>(pr.C, cn + pr.rowC[m], Cplx<T>{re, im});
+        store_elem<T, CP>(pr.C, cn + crow[i][r], Cplx<T>{re, im});
       }
     }
 }

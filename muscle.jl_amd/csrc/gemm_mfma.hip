@@ -220,13 +220,28 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   // partial sums of a split-K slice go to the workspace, [split][batch][m][n]
   const bool to_partial = p.ksplit > 1;
   const int64_t pbase = ((int64_t)blockIdx.z * gridDim.y + bz) * (int64_t)p.M * p.N;
+  // C's offsets for this thread's rows and columns, read before any store
+  // (a table load after a store to C could not be hoisted: they may alias)
+  int64_t crow[2][4], ccol[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wm * 32 + i * 16 + M_::row(lane, r);
+      crow[i][r] = (!to_partial && m < p.M) ? p.rowC[m] : 0;
+    }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+    ccol[j] = n < p.N ? (to_partial ? pbase + n : offC0 + p.colC[n]) : 0;
+  }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int n = n0 + wn * 32 + j * 16 + (lane & 15);
       if (n >= p.N) continue;
-      const int64_t cn = to_partial ? pbase + n : offC0 + p.colC[n];
+      const int64_t cn = ccol[j];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * 32 + i * 16 + M_::row(lane, r);
@@ -240,7 +255,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
           im = acc1[i][j][r];
         }
         if (to_partial) store_elem<T, CC>(p.partial, cn + (int64_t)m * p.N, Cplx<T>{re, im});
-        else store_elem<S, CC>(p.C, cn + p.rowC[m], Cplx<S>{(S)re, (S)im});
+        else store_elem<S, CC>(p.C, cn + crow[i][r], Cplx<S>{(S)re, (S)im});
       }
     }
 }
