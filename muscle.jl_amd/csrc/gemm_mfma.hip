@@ -20,7 +20,10 @@
 // buffered, so one barrier per K-tile suffices.  The global gather of K-tile
 // t+1 and the offset-table reads of K-tile t+2 are in flight while K-tile t
 // is multiplied.  Rows of 64 + 16 elements put the four 16-lane groups of an
-// MFMA operand read in disjoint LDS banks.  Workgroups are renumbered so each
+// MFMA operand read in disjoint LDS banks, and the column of element (k, m)
+// is m ^ (k & 15) (a permutation inside each aligned 16-block): the 16 lanes
+// that store 16 consecutive k of one row (operands contiguous along K) then
+// hit 16 different banks instead of one.  Workgroups are renumbered so each
 // XCD (the dispatcher deals workgroups round-robin over 8 XCDs) gets a
 // contiguous, grouped range of tiles and reuses A and B panels in its own L2.
 //
@@ -143,10 +146,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   auto stash = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
-      As[buf][0][a_kk[i]][a_mm[i]] = ra[i].re;
-      if constexpr (CA) As[buf][PA - 1][a_kk[i]][a_mm[i]] = ra[i].im;
-      Bs[buf][0][b_kk[i]][b_nn[i]] = rb[i].re;
-      if constexpr (CB) Bs[buf][PB - 1][b_kk[i]][b_nn[i]] = rb[i].im;
+      const int ca = a_mm[i] ^ (a_kk[i] & 15), cb = b_nn[i] ^ (b_kk[i] & 15);
+      As[buf][0][a_kk[i]][ca] = ra[i].re;
+      if constexpr (CA) As[buf][PA - 1][a_kk[i]][ca] = ra[i].im;
+      Bs[buf][0][b_kk[i]][cb] = rb[i].re;
+      if constexpr (CB) Bs[buf][PB - 1][b_kk[i]][cb] = rb[i].im;
     }
   };
 
@@ -178,10 +182,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
       T ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const int mm = wm * 32 + i * 16 + (lane & 15);
+        const int mm = (wm * 32 + i * 16 + (lane & 15)) ^ (kr & 15);
         ar[i] = As[cur][0][kr][mm];
         ai[i] = CA ? As[cur][PA - 1][kr][mm] : T(0);
-        const int nn = wn * 32 + i * 16 + (lane & 15);
+        const int nn = (wn * 32 + i * 16 + (lane & 15)) ^ (kr & 15);
         br[i] = Bs[cur][0][kr][nn];
         bi[i] = CB ? Bs[cur][PB - 1][kr][nn] : T(0);
       }
