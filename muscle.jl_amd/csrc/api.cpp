@@ -403,8 +403,51 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
 
 }  // namespace
 
+namespace {
+
+// Workspace of one contraction: casts | packed A | packed B | partial sums or
+// split-precision state, at these offsets.
+struct WsPlan {
+  size_t pack_a_off, pack_b_off, partial_off, split_off, need;
+  size_t es_a, es_b;
+};
+
+WsPlan workspace_plan(const Plan& p, const muscle_hip_tensor_t* ka, const muscle_hip_tensor_t* kb) {
+  const ContractionDesc& d = p.d;
+  const bool dbl = is_double(d.dt[2]);
+  WsPlan w;
+  size_t need = cast_bytes(*ka, dbl) + (kb ? cast_bytes(*kb, dbl) : 0);
+  const size_t csize = dtype_size(d.dt[2]);
+  // packed operand copies (element size after the precision promotion)
+  w.es_a = dtype_size(make_dtype(dbl, is_complex(ka->dtype)));
+  w.es_b = kb ? dtype_size(make_dtype(dbl, is_complex(kb->dtype))) : 0;
+  w.pack_a_off = need;
+  if (p.pack_a) need += align_up((size_t)(d.M * d.K * d.B) * w.es_a);
+  w.pack_b_off = need;
+  if (p.pack_b) need += align_up((size_t)(d.N * d.K * d.B) * w.es_b);
+  const Plan& q = p.inner ? *p.inner : p;
+  w.partial_off = need;
+  if (p.kind == Plan::REDUCE && p.nsplit > 1) need += align_up((size_t)(d.M * d.N * d.B) * csize * p.nsplit);
+  if (p.kind == Plan::GEMM && q.gemm_ksplit > 1)
+    need += align_up((size_t)(d.M * d.N * d.B) * csize * q.gemm_ksplit);
+  w.split_off = need;
+  if (p.kind == Plan::SPLIT) need += split_workspace_bytes(p);
+  w.need = need;
+  return w;
+}
+
+}  // namespace
+
+size_t contraction_workspace_bytes(const Plan& p, const muscle_hip_tensor_t& a, const muscle_hip_tensor_t* b) {
+  if (p.kind == Plan::EMPTY) return 0;
+  const muscle_hip_tensor_t* ka = &a;
+  const muscle_hip_tensor_t* kb = b;
+  if (p.d.swapped) std::swap(ka, kb);
+  return workspace_plan(p, ka, kb).need;
+}
+
 void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const muscle_hip_tensor_t& a,
-                     const muscle_hip_tensor_t* b) {
+                     const muscle_hip_tensor_t* b, char* ws, size_t ws_size) {
   if (p.kind == Plan::EMPTY) return;
   const ContractionDesc& d = p.d;
   const bool dbl = is_double(d.dt[2]);
@@ -412,29 +455,24 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   const muscle_hip_tensor_t* ka = &a;
   const muscle_hip_tensor_t* kb = b;
   if (d.swapped) std::swap(ka, kb);
-  size_t need = cast_bytes(*ka, dbl) + (kb ? cast_bytes(*kb, dbl) : 0);
   const size_t csize = dtype_size(d.dt[2]);
-  // packed operand copies (element size after the precision promotion)
-  const size_t es_a = dtype_size(make_dtype(dbl, is_complex(ka->dtype)));
-  const size_t es_b = kb ? dtype_size(make_dtype(dbl, is_complex(kb->dtype))) : 0;
-  const size_t pack_a_off = need;
-  if (p.pack_a) need += align_up((size_t)(d.M * d.K * d.B) * es_a);
-  const size_t pack_b_off = need;
-  if (p.pack_b) need += align_up((size_t)(d.N * d.K * d.B) * es_b);
+  const WsPlan w = workspace_plan(p, ka, kb);
+  const size_t es_a = w.es_a, es_b = w.es_b;
   Plan& q = p.inner ? *p.inner : p;  // the plan whose tables the GEMM reads
-  size_t partial_off = need;
-  if (p.kind == Plan::REDUCE && p.nsplit > 1) need += align_up((size_t)(d.M * d.N * d.B) * csize * p.nsplit);
-  if (p.kind == Plan::GEMM && q.gemm_ksplit > 1)
-    need += align_up((size_t)(d.M * d.N * d.B) * csize * q.gemm_ksplit);
-  size_t split_off = need;
-  if (p.kind == Plan::SPLIT) need += split_workspace_bytes(p);
-  if (need) h.ensure_workspace(need);
-  char* cursor = (char*)h.ws;
+  char* base;
+  if (ws) {  // a caller-provided workspace (e.g. inside a captured graph): no allocation here
+    if (w.need > ws_size) fail(MUSCLE_HIP_INVALID_ARGUMENT, "contraction workspace too small");
+    base = ws;
+  } else {
+    if (w.need) h.ensure_workspace(w.need);
+    base = (char*)h.ws;
+  }
+  char* cursor = base;
   int dta, dtb = d.dt[1];
   const void* A = promote_operand(h, *ka, dbl, cursor, dta);
   const void* B = kb ? promote_operand(h, *kb, dbl, cursor, dtb) : nullptr;
   if (p.kind == Plan::SPLIT) {
-    run_split(h, p, c.data, A, B, is_complex(dta), is_complex(dtb), (char*)h.ws + split_off);
+    run_split(h, p, c.data, A, B, is_complex(dta), is_complex(dtb), base + w.split_off);
     check_launch();
     return;
   }
@@ -447,7 +485,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   }
   if (p.kind == Plan::GEMM) {
     if (p.pack_a) {
-      char* buf = (char*)h.ws + pack_a_off;
+      char* buf = base + w.pack_a_off;
       launch_pack_operand((int)es_a, PackOpArgs{A, buf, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), (int32_t)d.M,
                                                 (int32_t)d.K, (int32_t)d.B, p.pack_a_kfast},
                           h.stream);
@@ -455,7 +493,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
       A = buf;
     }
     if (p.pack_b) {
-      char* buf = (char*)h.ws + pack_b_off;
+      char* buf = base + w.pack_b_off;
       launch_pack_operand((int)es_b, PackOpArgs{B, buf, p.tab(TAB_COLB), p.tab(TAB_KB), p.tab(TAB_BB), (int32_t)d.N,
                                                 (int32_t)d.K, (int32_t)d.B, p.pack_b_kfast},
                           h.stream);
@@ -481,7 +519,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     g.complex3m = h.complex_mult == 3;
     g.ksplit = q.gemm_ksplit;
     g.kchunk = q.gemm_kchunk;
-    g.partial = (char*)h.ws + partial_off;
+    g.partial = base + w.partial_off;
     // grid.y carries the batch; split launches beyond the 65535 limit
     for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
       g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);
@@ -498,7 +536,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   r.A = A;
   r.B = B;
   r.C = c.data;
-  r.partial = (char*)h.ws + partial_off;
+  r.partial = base + w.partial_off;
   r.rowA = p.tab(TAB_ROWA);
   r.kA = p.tab(TAB_KA);
   r.kB = p.tab(TAB_KB);
@@ -529,12 +567,8 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
   check_launch();
 }
 
-Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
-               const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b) {
-  std::string key = plan_key(kind, precision, c, a, b);
-  key.append((const char*)&target, sizeof target);
-  Plan* p = h.find_plan(key);
-  if (p) return p;
+std::unique_ptr<Plan> make_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
+                                const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b) {
   ContractionDesc d = kind == 1 ? lower_binary(*c, *a, *b) : lower_unary(*c, *a);
   std::unique_ptr<Plan> np = build_contraction_plan(d, precision);
   if (np->kind == Plan::SPLIT) prepare_split(*np, target);
@@ -542,7 +576,16 @@ Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_h
   if (np->kind != Plan::EMPTY) upload_tables(*np, h.stream);
   if (np->inner) upload_tables(*np->inner, h.stream);
   upload_split_tables(*np, h.stream);
-  return h.insert_plan(key, std::move(np));
+  return np;
+}
+
+Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
+               const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b) {
+  std::string key = plan_key(kind, precision, c, a, b);
+  key.append((const char*)&target, sizeof target);
+  Plan* p = h.find_plan(key);
+  if (p) return p;
+  return h.insert_plan(key, make_plan(h, kind, precision, target, c, a, b));
 }
 
 // ---- elementwise planning

@@ -71,8 +71,20 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
+  // the largest pairs run as single-contraction launches (packed operands,
+  // the plain GEMM tiles) inside the same graph, after the persistent launch;
+  // they own their plans and one workspace
+  std::vector<char> solo;                        // per pair
+  std::vector<int64_t> solo_pairs;               // their indices
+  std::vector<std::unique_ptr<Plan>> solo_plans;
+  void* solo_ws = nullptr;
+  size_t solo_ws_size = 0;
   ~Batch();
 };
+
+// Pairs of at least this many index points (M * N * K * batch) leave the
+// persistent launch for a single-contraction launch of their own.
+constexpr double kBatchSoloPoints = double(int64_t(1) << 29);
 
 // Location of one two-level table inside a batch's table arena.
 struct TabRef {
@@ -91,6 +103,7 @@ struct PlannedPair {
   const void* b = nullptr;
   void* c = nullptr;
   const int64_t* dtab = nullptr;  // device tables of this pair; null: the batch's own table region
+  bool solo = false;              // runs as a single-contraction launch, not in the work list
 };
 
 // Device layout of a planned batch: pairs | items | counter | tables.

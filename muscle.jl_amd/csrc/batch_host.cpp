@@ -16,11 +16,12 @@ namespace mh {
 
 Batch::~Batch() {
   // batches with their own device memory may still be read by queued work
-  if (h && (dmem || exec)) (void)hipStreamSynchronize(h->stream);
+  if (h && (dmem || exec || solo_ws)) (void)hipStreamSynchronize(h->stream);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (capture_stream) (void)hipStreamDestroy(capture_stream);
   if (dmem) (void)hipFree(dmem);
+  if (solo_ws) (void)hipFree(solo_ws);
 }
 
 namespace {
@@ -115,6 +116,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     const PlannedPair& L = *lw[(size_t)i];
     if (L.d.empty) continue;
     B.flops += (cp ? 8.0 : 2.0) * (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B;
+    if (L.solo) continue;
     const TileShape& ts = kTileShapes[L.tile];
     const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
     const int64_t items = tm * tn * L.d.B;
@@ -150,7 +152,9 @@ void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscl
       fail(MUSCLE_HIP_UNSUPPORTED, "batched contractions need one element type for every operand");
     PlannedPair& L = lw[(size_t)i];
     L.d = lower_binary(cs[i], as[i], bs[i]);
-    plan_pair(h, L, B.htab, tiles ? tiles[i] : MUSCLE_HIP_TILE_AUTO);
+    L.solo = keep_descs && !L.d.empty &&
+             (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B >= kBatchSoloPoints;
+    if (!L.solo) plan_pair(h, L, B.htab, tiles ? tiles[i] : MUSCLE_HIP_TILE_AUTO);
     L.a = L.d.swapped ? bs[i].data : as[i].data;
     L.b = L.d.swapped ? as[i].data : bs[i].data;
     L.c = cs[i].data;
@@ -203,6 +207,7 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
     const PlannedPair& L = *lw[(size_t)i];
     BatchPair& P = B.hpairs[(size_t)i];
     std::memset(&P, 0, sizeof P);
+    if (L.solo) continue;  // no tables, never in the work list
     P.A = L.a;
     P.B = L.b;
     P.C = L.c;
@@ -250,6 +255,24 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
   std::vector<const PlannedPair*> ptrs((size_t)n);
   for (int64_t i = 0; i < n; ++i) ptrs[(size_t)i] = &lw[(size_t)i];
   place_batch(*B, ptrs, (char*)B->dmem, nullptr, nullptr);
+  // the largest pairs: plans of their own and one workspace for all of them
+  B->solo.assign((size_t)n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (!lw[(size_t)i].solo) continue;
+    B->solo[(size_t)i] = 1;
+    B->solo_pairs.push_back(i);
+    B->solo_plans.push_back(make_plan(h, 1, MUSCLE_HIP_PRECISION_NATIVE, 1e-12, &B->cs[(size_t)i], &B->as[(size_t)i],
+                                      &B->bs[(size_t)i]));
+    B->solo_ws_size = std::max(B->solo_ws_size, contraction_workspace_bytes(*B->solo_plans.back(), B->as[(size_t)i],
+                                                                            &B->bs[(size_t)i]));
+  }
+  if (B->solo_ws_size) {
+    if (hipMalloc(&B->solo_ws, B->solo_ws_size) != hipSuccess) {
+      (void)hipGetLastError();
+      B->solo_ws = nullptr;
+      fail(MUSCLE_HIP_OUT_OF_MEMORY, "batch workspace allocation failed");
+    }
+  }
   return B;
 }
 
@@ -274,10 +297,29 @@ void batch_run(Batch& b, int mode) {
     }
     return;
   }
-  if (b.hitems.empty()) return;
+  if (b.hitems.empty() && b.solo_plans.empty()) return;
+  // the persistent launch, then the solo pairs' launches, on stream s (the
+  // solo launches take the handle's stream for the duration)
+  auto enqueue = [&](hipStream_t s) {
+    if (!b.hitems.empty()) {
+      batch_launch(b, s);
+      MH_HIP_CHECK(hipGetLastError());
+    }
+    const hipStream_t saved = h.stream;
+    h.stream = s;
+    try {
+      for (size_t k = 0; k < b.solo_plans.size(); ++k) {
+        const size_t i = (size_t)b.solo_pairs[k];
+        run_contraction(h, *b.solo_plans[k], b.cs[i], b.as[i], &b.bs[i], (char*)b.solo_ws, b.solo_ws_size);
+      }
+    } catch (...) {
+      h.stream = saved;
+      throw;
+    }
+    h.stream = saved;
+  };
   if (mode == 2) {
-    batch_launch(b, h.stream);
-    MH_HIP_CHECK(hipGetLastError());
+    enqueue(h.stream);
     return;
   }
   if (!b.exec) {
@@ -285,7 +327,14 @@ void batch_run(Batch& b, int mode) {
     // replay the graph on the handle's stream
     if (!b.capture_stream) MH_HIP_CHECK(hipStreamCreateWithFlags(&b.capture_stream, hipStreamNonBlocking));
     MH_HIP_CHECK(hipStreamBeginCapture(b.capture_stream, hipStreamCaptureModeThreadLocal));
-    batch_launch(b, b.capture_stream);
+    try {
+      enqueue(b.capture_stream);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(b.capture_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b.capture_stream, &g);
     if (e != hipSuccess) fail(MUSCLE_HIP_HIP_ERROR, std::string("graph capture: ") + hipGetErrorString(e));
@@ -301,7 +350,7 @@ void batch_autotune(Batch& b, int samples, int reps) {
   std::vector<std::vector<int64_t>> by_class(Handle::kTuneClasses);
   for (size_t i = 0; i < b.hpairs.size(); ++i) {
     const BatchPair& P = b.hpairs[i];
-    if (P.M * (int64_t)P.N * P.NB == 0) continue;
+    if (P.M * (int64_t)P.N * P.NB == 0 || (i < b.solo.size() && b.solo[i])) continue;
     by_class[tuning_class(P.M, P.N)].push_back((int64_t)i);
   }
   hipEvent_t e0, e1;

@@ -111,8 +111,15 @@ extern thread_local std::string g_last_error;
 
 std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int precision);
 void upload_tables(Plan& p, hipStream_t s);
+// Enqueues one planned contraction on h.stream.  Its workspace is the
+// handle's (grown as needed), or `ws` of `ws_size` bytes when given (no
+// allocation then: usable under graph capture).
 void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const muscle_hip_tensor_t& a,
-                     const muscle_hip_tensor_t* b);
+                     const muscle_hip_tensor_t* b, char* ws = nullptr, size_t ws_size = 0);
+size_t contraction_workspace_bytes(const Plan& p, const muscle_hip_tensor_t& a, const muscle_hip_tensor_t* b);
+// A plan that the caller owns (not in the handle's cache); kind 1 binary, 2 unary.
+std::unique_ptr<Plan> make_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
+                                const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b);
 Plan* get_plan(Handle& h, int kind, int precision, double target, const muscle_hip_tensor_t* c,
                const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b);
 // Thin SVD (factor.cpp) and the release of its rocBLAS handle and workspace.

@@ -14,7 +14,8 @@ namespace mh {
 struct FastDiv {
   uint32_t d, mul, shift;
   FastDiv() : d(1), mul(1), shift(0) {}
-  explicit FastDiv(uint32_t div) : d(div) {
+  explicit FastDiv(uint32_t div) : d(div ? div : 1) {
+    div = d;  // a divisor of 0 (an unused table) behaves as 1
     uint32_t l = 0;
     while ((uint64_t(1) << l) < div) ++l;
     shift = l;

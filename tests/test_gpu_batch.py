@@ -123,3 +123,44 @@ def test_batch_rejects_mixed_eltypes():
     c = muscle.Tensor(torch.zeros((4, 4), dtype=torch.complex128, device="cuda"), ["i", "k"])
     with pytest.raises(muscle.UnsupportedError):
         ContractionBatch([(c, a, b)])
+
+
+def test_largest_pairs_run_as_single_launches():
+    """Pairs of 2^29 or more index points leave the persistent launch for a
+    single-contraction launch of their own (packed operands, plain GEMM tiles)
+    inside the same graph; they match the oracle and, bit for bit, the plain loop."""
+    rng = np.random.default_rng(16)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=30)
+    small_items = ContractionBatch(pairs).info()["items"]
+    labels = [f"y{i}" for i in range(30)]
+    for s in range(2):
+        perm = list(rng.permutation(labels))
+        shared, left, right = perm[:10], perm[10:20], perm[20:]
+        ia = list(rng.permutation(left + shared))
+        ib = list(rng.permutation(shared + right))
+        ic = list(rng.permutation(left + right))
+        A, B = randn(rng, (2,) * 20, np.complex128), randn(rng, (2,) * 20, np.complex128)
+        a, b = dev_tensor(A, ia), dev_tensor(B, ib)
+        c = muscle.Tensor(muscle.empty_col_major([2] * 20, a.dtype, a.device), ic)
+        pairs.append((c, a, b))
+        refs.append(bb.binary_einsum_general(ic, A, ia, B, ib))
+    batch = ContractionBatch(pairs)
+    info = batch.info()
+    assert info["items"] == small_items  # the two 2^30-point pairs are not in the work list
+    assert info["flops"] >= 2 * 8.0 * 2.0 ** 30
+    batch.run("persistent_nograph")
+    torch.cuda.synchronize()
+    eager = [c.parent().clone() for c, a, b in pairs]
+    check(pairs, refs, 1e-12)
+    for rep in range(2):
+        for c, a, b in pairs:
+            c.parent().fill_(float("nan"))
+        batch.run("persistent")
+        torch.cuda.synchronize()
+        for (c, a, b), e in zip(pairs, eager):
+            assert torch.equal(c.parent(), e), rep
+    batch.run("loop")
+    torch.cuda.synchronize()
+    for (c, a, b), e in zip(pairs[-2:], eager[-2:]):
+        assert torch.equal(c.parent(), e)
+    batch.close()
