@@ -194,9 +194,13 @@ MUSCLE_HIP_API int muscle_hip_unary_einsum(muscle_hip_handle_t handle,
  * decreasing order.  V holds the conjugated right singular vectors, as the
  * reference's Vt.  U and V have A's element type, S its real type.
  * Replaces tensor_svd_thin(::Backend, A; inds_u, inds_v, ind_s)
- * (src/Operations/tensor_svd.jl:100-125; Julia LinearAlgebra.svd); the matrix
- * SVD is rocSOLVER's gesvd, the permutations are this library's copies.
- * Synchronizes the handle's stream once (the solver's convergence flag). */
+ * (src/Operations/tensor_svd.jl:100-125; Julia LinearAlgebra.svd).  The
+ * matrix SVD comes from the Hermitian eigenproblem of [[0, M], [M^H, 0]]
+ * (rocSOLVER syevd/heevd) with two Cholesky-QR passes that make U and V
+ * orthonormal; singular vectors of singular values at the solver's noise
+ * level are an orthonormal completion, as LAPACK's are for a zero singular
+ * value.  The permutations are this library's copies.  Synchronizes the
+ * handle's stream twice (the eigenvalues, then the solver flags). */
 MUSCLE_HIP_API int muscle_hip_svd_thin(muscle_hip_handle_t handle, const muscle_hip_tensor_t* u,
                                        const muscle_hip_tensor_t* s, const muscle_hip_tensor_t* v,
                                        const muscle_hip_tensor_t* a);

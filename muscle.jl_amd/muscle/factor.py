@@ -4,7 +4,7 @@ HIP backend.
 Front ends mirror src/Operations/tensor_svd.jl:59-68 and :160-215,
 src/Operations/tensor_qr.jl:27-68 (with factorinds, src/Index.jl:41-63) and
 src/Operations/simple_update.jl:16-87; the device work is muscle_hip_svd_thin
-(permutation copies + rocSOLVER gesvd), muscle_hip_qr_thin (permutation
+(permutation copies + the eigensolver SVD), muscle_hip_qr_thin (permutation
 copies + geqrf/orgqr) and this library's binary_einsum / hadamard!.
 
 The in-place forms tensor_svd_thin_(U, s, V, A) and tensor_qr_thin_(Q, R, A)
