@@ -19,6 +19,8 @@ and, for the two-site gate update that follows the contractions, the
 factorization behind it:
 
   tensor_svd_thin(::BackendHIP, A; inds_u, inds_v, ind_s)   (src/Operations/tensor_svd.jl:59-68)
+  tensor_svd_trunc(::BackendHIP, A; ..., threshold, maxdim)  (:160-215)
+  tensor_qr_thin(::BackendHIP, A; inds_q, inds_r, ind_virtual) (src/Operations/tensor_qr.jl:27-68)
   simple_update(::BackendHIP, A, ...)                        (src/Operations/simple_update.jl:37-87)
 
 It assumes `struct BackendHIP <: Backend end` next to the other backends
@@ -205,6 +207,42 @@ function Muscle.tensor_svd_thin(::BackendHIP, A::Tensor; inds_u=(), inds_v=(), i
                     handle(), du, ds, dv, da))
     end
     return U, s, Vt
+end
+
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_qr_thin), ::DomainROCm) = BackendHIP()
+
+# Q[inds_q..., ind_virtual], R[ind_virtual, inds_r...] with A = Q R
+# (src/Operations/tensor_qr.jl:45-68)
+function Muscle.tensor_qr_thin(::BackendHIP, A::Tensor; inds_q=(), inds_r=(), ind_virtual=Index(gensym(:qr)),
+                               kwargs...)
+    ind_virtual ∉ inds(A) || throw(ArgumentError("new virtual bond name ($ind_virtual) cannot be already be present"))
+    inds_q, inds_r = Muscle.factorinds(inds(A), inds_q, inds_r)
+    issetequal(inds_q ∪ inds_r, inds(A)) || throw(ArgumentError("inds_q and inds_r must cover inds(A)"))
+    sq = [size(A, i) for i in inds_q]
+    sr = [size(A, i) for i in inds_r]
+    k = min(prod(sq), prod(sr))
+    Q = Tensor(similar(parent(A), sq..., k), [inds_q; ind_virtual])
+    R = Tensor(similar(parent(A), k, sr...), [ind_virtual; inds_r])
+    labels = modemap(Q, R, A)
+    dq, kq = describe(Q, labels)
+    dr, kr = describe(R, labels)
+    da, ka = describe(A, labels)
+    GC.@preserve Q R A kq kr ka begin
+        check(ccall((:muscle_hip_qr_thin, LIB), Cint, (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}),
+                    handle(), dq, dr, da))
+    end
+    return Q, R
+end
+
+# the thin SVD, then the cut of src/Operations/tensor_svd.jl:183-203 on the host
+function Muscle.tensor_svd_trunc(::BackendHIP, A::Tensor; inds_u=(), inds_v=(), ind_s=Index(gensym(:vind)),
+                                 threshold=nothing, maxdim=nothing, kwargs...)
+    U, s, Vt = Muscle.tensor_svd_thin(BackendHIP(), A; inds_u, inds_v, ind_s)
+    sh = Array(parent(s))
+    k = length(sh)
+    isnothing(maxdim) || (k = min(k, maxdim))
+    isnothing(threshold) || (k = something(findfirst(<(sqrt(sum(abs2, sh)) * threshold), view(sh, 1:k)), k))
+    return view(U, ind_s => 1:k), view(s, ind_s => 1:k), view(Vt, ind_s => 1:k)
 end
 
 # simple_update.jl:37-87 on device tensors: the contractions and the absorption
