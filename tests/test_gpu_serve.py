@@ -13,7 +13,7 @@ import muscle
 from muscle.serve import LeafBank, Server, tree_nodes
 from muscle.workloads import contraction_tree
 from oracle import backend_base as bb
-from tests.gpu_util import host, rel_err
+from tests.gpu_util import dev_tensor, host, randn, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -178,3 +178,34 @@ def test_invalid_jobs():
     assert srv.wait(jid, timeout=120.0)["status"] == 0
     srv.release(jid)
     srv.close()
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_job_with_single_launch_nodes(dtype):
+    """A node of 2^30 index points runs as a single-contraction launch inside the
+    server's tick (the others in the persistent launch); the root matches the oracle."""
+    rng = np.random.default_rng(7)
+    a = [f"a{i}" for i in range(10)]
+    s = [f"s{i}" for i in range(10)]
+    b = [f"b{i}" for i in range(10)]
+    c = [f"c{i}" for i in range(5)]
+    specs = [(list(rng.permutation(a + s)), 20), (list(rng.permutation(s + b)), 20), (b[:5] + c, 10)]
+    vals, leaves = [], []
+    for labs, r in specs:
+        x = randn(rng, (2,) * r, dtype)
+        vals.append(x)
+        leaves.append(dev_tensor(x, [muscle.Index(l) for l in labs]))
+    mid = list(rng.permutation(a + b))
+    out = a + b[5:] + c
+    nodes = [(0, 1, [muscle.Index(x) for x in mid]), (3, 2, [muscle.Index(x) for x in out])]
+    srv = Server(pool_gib=2)
+    jid = srv.submit(leaves, nodes)
+    r = srv.wait(jid, timeout=120.0)
+    assert r["status"] == 0, r
+    got = host(srv.result(jid))
+    srv.release(jid)
+    srv.close()
+    m = bb.binary_einsum_general(mid, vals[0].astype(np.complex128), specs[0][0], vals[1].astype(np.complex128),
+                                 specs[1][0])
+    ref = bb.binary_einsum_general(out, m, mid, vals[2].astype(np.complex128), specs[2][0])
+    assert rel_err(got, ref) <= (1e-12 if dtype == np.complex128 else 2e-5)
