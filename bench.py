@@ -76,12 +76,18 @@ def timed(fn, steps: int, warmup: int):
     return e0.elapsed_time(e1) / steps, out
 
 
-def contraction_case(a, b, kw, backend, steps, warmup):
+def contraction_case(a, b, kw, backend, steps, warmup, settle_ms=0.0):
+    """ms per call over `steps` calls after `warmup` calls, plus, with settle_ms, enough
+    further untimed calls to fill about settle_ms (clocks settle after the previous case)."""
     import muscle
 
     def step():
         with muscle.with_backend(backend):
             return muscle.binary_einsum(a, b, **kw)
+    if settle_ms > 0:
+        est, _ = timed(step, 2, warmup)
+        timed(step, max(1, int(settle_ms / max(est, 1e-3))), 0)
+        warmup = 0
     ms, c = timed(step, steps, warmup)
     return ms, c, muscle.last_info()
 
@@ -573,13 +579,14 @@ def main() -> int:
     # ---------------- other configurations
     if want("c2_c32"):
         A32, ia2, B32, ib2, ic2 = wl.config2_inputs(16, seed=0, dtype=np.complex64)
-        m2, c2, _ = contraction_case(put(A32, ia2, dev), put(B32, ib2, dev), {"out": ic2}, plain, args.steps, 2)
+        m2, c2, _ = contraction_case(put(A32, ia2, dev), put(B32, ib2, dev), {"out": ic2}, plain, args.steps, 2, 100.0)
         extra["c2_c32"] = {"ms": round(m2, 4), "tflops": round(flops / (m2 * 1e-3) / 1e12, 3),
                            "frac_fp32_peak": round(flops / (m2 * 1e-3) / 1e12 / PEAK_TFLOPS["fp32"], 4)}
         del c2
     if want("c2h_c64"):
         Ah, iah, Bh, ibh, ich = wl.config2h_inputs(1, seed=0)
-        m2, c2, inf2 = contraction_case(put(Ah, iah, dev), put(Bh, ibh, dev), {"out": ich}, plain, args.steps, 2)
+        m2, c2, inf2 = contraction_case(put(Ah, iah, dev), put(Bh, ibh, dev), {"out": ich}, plain, args.steps, 2,
+                                        100.0)
         extra["c2h_c64"] = {"ms": round(m2, 4), "tflops": round(inf2["flops"] / (m2 * 1e-3) / 1e12, 3),
                             "batch": inf2["batch"], "m": inf2["m"], "n": inf2["n"], "k": inf2["k"]}
         del c2
@@ -598,7 +605,7 @@ def main() -> int:
             Bd = (rng.standard_normal(2 ** 24) + 1j * rng.standard_normal(2 ** 24)).astype(T) / 64
             ad = put(Ad.reshape((2,) * 24, order="F"), ia, dev)
             bd = put(Bd.reshape((2,) * 24, order="F"), ib, dev)
-            m2, c2, inf2 = contraction_case(ad, bd, {"out": ic}, plain, max(3, args.steps // 2), 2)
+            m2, c2, inf2 = contraction_case(ad, bd, {"out": ic}, plain, max(3, args.steps // 2), 2, 100.0)
             peak = PEAK_TFLOPS["fp64" if nm == "c64" else "fp32"]
             tf = inf2["flops"] / (m2 * 1e-3) / 1e12
             res[nm] = {"ms": round(m2, 4), "tflops": round(tf, 3), "frac_peak": round(tf / peak, 4)}
