@@ -126,6 +126,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
     b_ok[i] = n < p.N;
     rowA_r[i] = p.rowA[a_ok[i] ? m : p.M - 1] + offA0;
     colB_r[i] = p.colB[b_ok[i] ? n : p.N - 1] + offB0;
+    a_mm[i] ^= a_kk[i] & 15;  // from here on: the swizzled LDS columns
+    b_nn[i] ^= b_kk[i] & 15;
   }
   const int nk = (kend - kbeg + BK - 1) / BK;
 
@@ -139,7 +141,20 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
     }
   };
   Cplx<T> ra[NE], rb[NE];
+  // a tile whose rows and columns all lie inside M and N needs no per-element
+  // checks on the K-tiles that lie inside its K range (all but the last)
+  const bool full_mn = m0 + BM <= p.M && n0 + BN <= p.N;
   auto gather = [&](int kt) {
+    if (full_mn && kbeg + (kt + 1) * BK <= kend) {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        const Cplx<S> xa = load_elem<S, CA>(p.A, rowA_r[i] + tA[i]);
+        const Cplx<S> xb = load_elem<S, CB>(p.B, colB_r[i] + tB[i]);
+        ra[i] = Cplx<T>{(T)xa.re, (T)xa.im};
+        rb[i] = Cplx<T>{(T)xb.re, (T)xb.im};
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       const Cplx<S> xa = load_elem<S, CA>(p.A, rowA_r[i] + tA[i]);
@@ -154,11 +169,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   auto stash = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
-      const int ca = a_mm[i] ^ (a_kk[i] & 15), cb = b_nn[i] ^ (b_kk[i] & 15);
-      As[buf][0][a_kk[i]][ca] = ra[i].re;
-      if constexpr (CA) As[buf][PA - 1][a_kk[i]][ca] = ra[i].im;
-      Bs[buf][0][b_kk[i]][cb] = rb[i].re;
-      if constexpr (CB) Bs[buf][PB - 1][b_kk[i]][cb] = rb[i].im;
+      // a_mm / b_nn hold the swizzled LDS columns by now
+      As[buf][0][a_kk[i]][a_mm[i]] = ra[i].re;
+      if constexpr (CA) As[buf][PA - 1][a_kk[i]][a_mm[i]] = ra[i].im;
+      Bs[buf][0][b_kk[i]][b_nn[i]] = rb[i].re;
+      if constexpr (CB) Bs[buf][PB - 1][b_kk[i]][b_nn[i]] = rb[i].im;
     }
   };
 
