@@ -226,3 +226,17 @@ def test_unary_errors():
     x = dev_tensor(np.ones((2, 3)), ["i", "j"])
     with pytest.raises(muscle.ArgumentError):
         muscle.unary_einsum(x, out=["x"])
+
+
+def test_hadamard_tiled_strided_views():
+    """a and c non-contiguous views (i fast, strided columns), b with k fast."""
+    rng = np.random.default_rng(25)
+    X = randn(rng, (130, 3, 120), np.complex128)
+    B = randn(rng, (120, 130), np.complex128)
+    a = muscle.Tensor(dev_tensor(X, ["i", "s", "k"]).parent()[:, 1, :], ["i", "k"])
+    big = torch.zeros((120, 2, 130), dtype=torch.complex128, device="cuda")
+    c = muscle.Tensor(big[:, 1, :].t(), ["i", "k"])
+    muscle.hadamard_(c, a, dev_tensor(B, ["k", "i"]))
+    got = big.cpu().numpy()
+    assert rel_err(got[:, 1, :].T, X[:, 1, :] * B.T) <= 1e-15
+    assert not got[:, 0, :].any()

@@ -283,3 +283,33 @@ def test_in_place_forms():
     muscle.tensor_qr_thin_(Q, R, A)
     (Qo, iq), (Ro, ir) = of.tensor_qr_thin_base(A_np, ["p", "q", "r"], ["q", "r"], ["p"], "y")
     assert rel_err(host(R), Ro) <= 1e-10 and rel_err(host(Q), Qo) <= 1e-10
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (5, 1), (1, 7), (300, 1)])
+def test_factor_degenerate_shapes(shape):
+    """k = 1 factorizations: a column, a row, a single element, a tall vector
+    (QR-reduced before the eigensolver)."""
+    rng = np.random.default_rng(41)
+    A_np = randn(rng, shape)
+    A = dev_tensor(A_np, [I("a"), I("b")])
+    U, S, Vt = muscle.tensor_svd_thin(A, inds_u=[I("a")], ind_s=I("x"))
+    u, s, v = host(U), host(S), host(Vt)
+    assert s.shape == (1,) and np.isclose(s[0], np.linalg.norm(A_np), rtol=1e-13)
+    assert rel_err((u * s) @ v.T, A_np) <= 1e-13
+    Q, R = muscle.tensor_qr_thin(A, inds_q=[I("a")], ind_virtual=I("y"))
+    q, r = host(Q), host(R)
+    assert rel_err(q @ r, A_np) <= 1e-13 and np.isclose(np.linalg.norm(q), 1.0)
+
+
+def test_simple_update_maxdim_above_rank():
+    rng = np.random.default_rng(42)
+    a, b = randn(rng, (4, 2, 3)), randn(rng, (3, 2, 5))
+    g = randn(rng, (2, 2, 2, 2))
+    A = dev_tensor(a, [I("l"), I("pa"), I("bond")])
+    B = dev_tensor(b, [I("bond"), I("pb"), I("r")])
+    G = dev_tensor(g, [I("pa"), I("pb"), I("ga"), I("gb")])
+    U, S, V = muscle.simple_update(A, I("pa"), B, I("pb"), I("bond"), G, I("ga"), I("gb"), maxdim=100)
+    assert S.size(I("bond")) == 8  # min(4 * 2, 2 * 5)
+    (Uo, iu), (so, _), (Vo, iv) = of.simple_update_base(a, ["l", "pa", "bond"], "pa", b, ["bond", "pb", "r"], "pb",
+                                                        "bond", g, ["pa", "pb", "ga", "gb"], "ga", "gb", maxdim=100)
+    assert rel_err(host(S), so) <= 1e-12
