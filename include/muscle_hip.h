@@ -233,12 +233,13 @@ typedef enum {
 
 typedef struct {
   int64_t pairs;      /* contractions in the batch */
-  int64_t items;      /* output tiles the persistent launch works through */
+  int64_t items;      /* output tiles of the batched launches */
   int32_t workgroups; /* persistent grid size */
   int32_t graph;      /* 1 once the launch has been captured in a hipGraph */
   double flops;       /* sum of algorithmic flops */
   double bytes;       /* sum of algorithmic bytes (every operand element once) */
   int64_t table_bytes;/* device bytes of offset tables and pair descriptors */
+  int64_t shallow_items;/* of `items`, the tiles of pairs with K <= 16 (vector-FMA kernel) */
 } muscle_hip_batch_info_t;
 
 /* Plans `n` independent binary contractions C_i = A_i * B_i, each with the

@@ -14,6 +14,8 @@
 #include "batch.hpp"
 #include "device_types.hpp"
 
+#include <algorithm>
+
 namespace mh {
 namespace {
 
@@ -250,6 +252,155 @@ __global__ __launch_bounds__(NT, kBatchWgPerCu) void batch_kernel(BatchArgs a) {
   }
 }
 
+// ---- shallow pairs --------------------------------------------------------
+// A pair whose contracted extent K is at most kShallowK does at most
+// 2 * kShallowK real flops per complex output element it writes, so it is
+// bound by writing C; the MFMA tile above would spend its time waiting on
+// its chain of dependent table and data loads at 2 workgroups per CU.  Here
+// one workgroup of 512 threads writes one output tile (the pair's tile shape,
+// BM x BN = 4096 elements) with vector FMAs and few registers, so several
+// workgroups share a CU and hide each other's latency:
+//   1. every offset the tile needs (rows and columns of A, B and C, the K
+//      offsets, the batch offsets) is looked up once, one lookup per thread,
+//      into LDS;
+//   2. per pass of SKC K elements, A's BM x SKC and B's SKC x BN elements are
+//      gathered into LDS (k-major, so either operand is read along its row
+//      or column without bank conflicts);
+//   3. each thread owns 8 outputs along one line of C: along a row of C
+//      (fixed m, 8 columns) when C is contiguous along M, else along a
+//      column; its own operand element is read once per k and the other
+//      operand's 8 elements are wave-wide LDS broadcasts;
+//   4. C is stored with consecutive lanes on consecutive addresses of C's
+//      contiguous direction.
+// Each output sums its K products in increasing k with the complex product
+// a * b evaluated the same way in both orientations, so a pair's result does
+// not depend on the orientation or on what else is in the batch.
+constexpr int SKC = 8;                // K elements staged per pass
+constexpr int SMAXE = 160;            // BM + BN of every tile shape
+constexpr int SNT = 512;              // threads per workgroup
+constexpr int SOUT = 4096 / SNT;      // outputs per thread
+constexpr int SGA = (128 * SKC + SNT - 1) / SNT;  // gathered elements per operand and thread
+constexpr int SOFF_ROWA = 0, SOFF_ROWC = 128, SOFF_COLB = 256, SOFF_COLC = 384, SOFF_KA = 512,
+              SOFF_KB = SOFF_KA + kShallowK, SOFF_BATCH = SOFF_KB + kShallowK, SOFF_N = SOFF_BATCH + 3;
+
+template <typename T, bool CP>
+__device__ inline void cmac(Cplx<T>& acc, const Cplx<T>& a, const Cplx<T>& b) {
+  acc.re = fma(a.re, b.re, acc.re);
+  if constexpr (CP) {
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+  }
+}
+
+template <typename T, bool CP, bool CM>
+__device__ __forceinline__ void shallow_tile(const BatchPair& pr, int BM, int BN, int mr, int nr, Cplx<T>* sab,
+                                             const int64_t* soff) {
+  const int tid = threadIdx.x;
+  Cplx<T>* sA = sab;             // [k][m], BM per k
+  Cplx<T>* sB = sab + BM * SKC;  // [k][n], BN per k
+  const int64_t offA = soff[SOFF_BATCH], offB = soff[SOFF_BATCH + 1];
+  // this thread's line of C: fixed index f along C's contiguous direction,
+  // SOUT indices v0 + vs * i across it
+  const int LF = CM ? BM : BN;
+  const int f = tid & (LF - 1), v0 = tid / LF, vs = SNT / LF;
+  Cplx<T> acc[SOUT];
+#pragma unroll
+  for (int i = 0; i < SOUT; ++i) acc[i] = Cplx<T>{T(0), T(0)};
+  const int K = pr.K;
+  for (int k0 = 0; k0 < K; k0 += SKC) {
+    const int kn = min(SKC, K - k0);
+    if (k0 > 0) __syncthreads();  // readers of the previous pass are done
+    // gather: at most SGA elements of A and SGA of B per thread, all loads first
+    Cplx<T> ra[SGA], rb[SGA];
+#pragma unroll
+    for (int j = 0; j < SGA; ++j) {
+      const int e = tid + SNT * j;
+      ra[j] = rb[j] = Cplx<T>{T(0), T(0)};
+      if (e < BM * SKC) {
+        const int m = pr.a_kmajor ? e / SKC : e & (BM - 1), k = pr.a_kmajor ? e & (SKC - 1) : e / BM;
+        if (m < mr && k < kn) ra[j] = load_elem<T, CP>(pr.A, offA + soff[SOFF_ROWA + m] + soff[SOFF_KA + k0 + k]);
+      }
+      if (e < BN * SKC) {
+        const int n = pr.b_kmajor ? e / SKC : e & (BN - 1), k = pr.b_kmajor ? e & (SKC - 1) : e / BN;
+        if (n < nr && k < kn) rb[j] = load_elem<T, CP>(pr.B, offB + soff[SOFF_COLB + n] + soff[SOFF_KB + k0 + k]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < SGA; ++j) {
+      const int e = tid + SNT * j;
+      if (e < BM * SKC) {
+        const int m = pr.a_kmajor ? e / SKC : e & (BM - 1), k = pr.a_kmajor ? e & (SKC - 1) : e / BM;
+        sA[k * BM + m] = ra[j];
+      }
+      if (e < BN * SKC) {
+        const int n = pr.b_kmajor ? e / SKC : e & (BN - 1), k = pr.b_kmajor ? e & (SKC - 1) : e / BN;
+        sB[k * BN + n] = rb[j];
+      }
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kn; ++kk) {
+      if constexpr (CM) {
+        const Cplx<T> a = sA[kk * BM + f];
+#pragma unroll
+        for (int i = 0; i < SOUT; ++i) cmac<T, CP>(acc[i], a, sB[kk * BN + v0 + vs * i]);
+      } else {
+        const Cplx<T> b = sB[kk * BN + f];
+#pragma unroll
+        for (int i = 0; i < SOUT; ++i) cmac<T, CP>(acc[i], sA[kk * BM + v0 + vs * i], b);
+      }
+    }
+  }
+  const int64_t offC = soff[SOFF_BATCH + 2];
+#pragma unroll
+  for (int i = 0; i < SOUT; ++i) {
+    const int m = CM ? f : v0 + vs * i, n = CM ? v0 + vs * i : f;
+    if (m < mr && n < nr) store_elem<T, CP>(pr.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
+  }
+}
+
+// 6 waves per SIMD: three 8-wave workgroups per CU (80 registers, no spills)
+template <typename T, bool CP>
+__global__ __launch_bounds__(SNT) __attribute__((amdgpu_waves_per_eu(6, 8))) void shallow_kernel(
+    ShallowArgs a) {
+  __shared__ Cplx<T> sab[SMAXE * SKC];
+  __shared__ int64_t soff[SOFF_N];
+  const int tid = threadIdx.x;
+  for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const int2 w = a.items[it];
+    const BatchPair& pr = a.pairs[w.x];
+    const int BM = pr.tile == MUSCLE_HIP_TILE_32x128 ? 32 : pr.tile == MUSCLE_HIP_TILE_128x32 ? 128 : 64;
+    const int BN = 4096 / BM;
+    const int per_batch = pr.tiles_m * pr.tiles_n;
+    const int bz = w.y / per_batch;
+    const int r = w.y - bz * per_batch;
+    const int tm = r / pr.tiles_n, tn = r - tm * pr.tiles_n;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int mr = min(BM, pr.M - m0), nr = min(BN, pr.N - n0), K = pr.K;
+    __syncthreads();  // the previous item's readers of LDS are done
+    // one table lookup per thread: rows, columns, K offsets, batch offsets
+    const int e = tid;
+    if (e < BM) {
+      soff[SOFF_ROWA + e] = e < mr ? pr.rowA[m0 + e] : 0;
+      soff[SOFF_ROWC + e] = e < mr ? pr.rowC[m0 + e] : 0;
+    } else if (e < BM + BN) {
+      const int n = e - BM;
+      soff[SOFF_COLB + n] = n < nr ? pr.colB[n0 + n] : 0;
+      soff[SOFF_COLC + n] = n < nr ? pr.colC[n0 + n] : 0;
+    } else if (e < BM + BN + K) {
+      const int k = e - BM - BN;
+      soff[SOFF_KA + k] = pr.kA[k];
+      soff[SOFF_KB + k] = pr.kB[k];
+    } else if (e < BM + BN + K + 3) {
+      const int j = e - BM - BN - K;
+      soff[SOFF_BATCH + j] = j == 0 ? pr.bA[bz] : j == 1 ? pr.bB[bz] : pr.bC[bz];
+    }
+    __syncthreads();
+    if (pr.c_mfast) shallow_tile<T, CP, true>(pr, BM, BN, mr, nr, sab, soff);
+    else shallow_tile<T, CP, false>(pr, BM, BN, mr, nr, sab, soff);
+  }
+}
+
 template <typename T, bool CP, bool G3>
 void launch_t(const BatchArgs& a, int wg, size_t lds, hipStream_t s) {
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
@@ -278,6 +429,19 @@ size_t batch_lds_bytes(bool dbl, bool cplx, int tile) {
   const size_t el = dbl ? 8 : 4, pl = cplx ? 2 : 1;
   const size_t bufs = tile == MUSCLE_HIP_TILE_64x64 ? 2 : 1;  // the double-buffered shape
   return bufs * pl * BK * ((size_t)(t.bm + LDP) + (t.bn + LDP)) * el;
+}
+
+void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s) {
+  if (a.nitems <= 0) return;
+  // a grid-stride loop over the items; 2^20 workgroups fill the GPU many times
+  const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, int64_t(1) << 20);
+  if (dbl) {
+    if (cplx) hipLaunchKernelGGL((shallow_kernel<double, true>), dim3(grid), dim3(SNT), 0, s, a);
+    else hipLaunchKernelGGL((shallow_kernel<double, false>), dim3(grid), dim3(SNT), 0, s, a);
+  } else {
+    if (cplx) hipLaunchKernelGGL((shallow_kernel<float, true>), dim3(grid), dim3(SNT), 0, s, a);
+    else hipLaunchKernelGGL((shallow_kernel<float, false>), dim3(grid), dim3(SNT), 0, s, a);
+  }
 }
 
 void launch_batch(bool dbl, bool cplx, const BatchArgs& a, int wg, size_t lds, hipStream_t s) {

@@ -18,7 +18,8 @@ struct BatchPair {
   const void* B;
   void* C;
   int32_t M, N, K, NB;  // fused extents and batch (hyper) count
-  int32_t tiles_m, tiles_n, tile, a_kmajor, b_kmajor, pad;
+  int32_t tiles_m, tiles_n, tile, a_kmajor, b_kmajor;
+  int32_t c_mfast;  // C's contiguous direction is M (else N)
   OffTab rowA, kA, kB, colB, rowC, colC, bA, bB, bC;
 };
 
@@ -38,6 +39,19 @@ struct BatchArgs {
   int32_t item_off;   // byte offset of the broadcast work item in dynamic LDS
 };
 
+// Pairs whose contracted extent K is at most this run in the shallow kernel
+// (vector FMAs, one output tile per workgroup, several workgroups per CU)
+// instead of the persistent MFMA launch.
+constexpr int kShallowK = 16;
+
+struct ShallowArgs {
+  const BatchPair* pairs;
+  const int2* items;  // the shallow pairs' tiles
+  int64_t nitems;
+};
+
+// Launches the shallow kernel over a.nitems tiles (nothing when there are none).
+void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s);
 // Launches the persistent kernel (grid = workgroups) preceded by a one-thread
 // kernel that resets the work counter (no memset nodes: the launch is graph
 // captured).  lds_bytes: dynamic LDS for the largest tile shape present.
@@ -58,7 +72,8 @@ struct Batch {
   std::vector<std::vector<int32_t>> modes;      // storage behind cs/as/bs
   std::vector<std::vector<int64_t>> ext, str;
   std::vector<BatchPair> hpairs;
-  std::vector<int2> hitems;
+  std::vector<int2> hitems;  // persistent items (largest K first), then shallow items
+  int64_t nshallow = 0;       // the shallow items at the end of hitems
   std::vector<int64_t> htab;
   std::vector<int> pair_class;
   void* dmem = nullptr;  // pairs | items | counter | tables
@@ -97,7 +112,7 @@ struct PlannedPair {
   ContractionDesc d;
   TabRef t[9];
   int tile = 0;
-  int a_kmajor = 0, b_kmajor = 0;
+  int a_kmajor = 0, b_kmajor = 0, c_mfast = 0;
   int64_t cost = 0;
   const void* a = nullptr;  // kernel operands (after the lowering's A/B exchange)
   const void* b = nullptr;
@@ -124,7 +139,8 @@ void plan_pair(const Handle& h, PlannedPair& L, std::vector<int64_t>& htab, int 
 // points on, which keeps the tables (host time and upload) small at one more
 // lookup per index.
 constexpr int64_t kTickSingleMax = 256;
-// Work list (largest K first), workgroup count and LDS size of the pairs in lw.
+// Work list (the persistent items, largest K first, then the shallow pairs'
+// items), workgroup count and LDS size of the pairs in lw.
 void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw);
 BatchLayout batch_layout(const Batch& B, int64_t n);
 void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base, char* staging, hipStream_t s);

@@ -99,6 +99,7 @@ void plan_pair(const Handle& h, PlannedPair& L, std::vector<int64_t>& htab, int 
   };
   L.a_kmajor = minst(d.k, 0) < minst(d.m, 0);
   L.b_kmajor = minst(d.k, 1) < minst(d.n, 1);
+  L.c_mfast = minst(d.m, 2) < minst(d.n, 2);
 }
 
 void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw) {
@@ -112,6 +113,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
                    [&](int64_t x, int64_t y) { return lw[(size_t)x]->cost > lw[(size_t)y]->cost; });
   bool used[MUSCLE_HIP_TILE_COUNT] = {false, false, false};
   const bool cp = is_complex(B.dtype);
+  std::vector<int2> shallow;
   for (int64_t i : order) {
     const PlannedPair& L = *lw[(size_t)i];
     if (L.d.empty) continue;
@@ -121,9 +123,13 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
     const int64_t items = tm * tn * L.d.B;
     if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
-    for (int64_t t = 0; t < items; ++t) B.hitems.push_back(make_int2((int)i, (int)t));
-    used[L.tile] = true;
+    const bool sh = L.d.K <= kShallowK;
+    std::vector<int2>& dst = sh ? shallow : B.hitems;
+    for (int64_t t = 0; t < items; ++t) dst.push_back(make_int2((int)i, (int)t));
+    if (!sh) used[L.tile] = true;
   }
+  B.nshallow = (int64_t)shallow.size();
+  B.hitems.insert(B.hitems.end(), shallow.begin(), shallow.end());
   int cus = h.cus;
   if (cus <= 0) {
     MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
@@ -221,6 +227,7 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
     P.tile = L.tile;
     P.a_kmajor = L.a_kmajor;
     P.b_kmajor = L.b_kmajor;
+    P.c_mfast = L.c_mfast;
     OffTab* tabs[9] = {&P.rowA, &P.colB, &P.kA, &P.kB, &P.rowC, &P.colC, &P.bA, &P.bB, &P.bC};
     // tables were appended in the order rowA, kA, kB, colB, rowC, colC, bA, bB, bC
     const int map[9] = {0, 3, 1, 2, 4, 5, 6, 7, 8};
@@ -277,15 +284,25 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
 }
 
 void batch_launch(Batch& b, hipStream_t s) {
-  BatchArgs a;
-  a.pairs = b.dpairs;
-  a.items = b.ditems;
-  a.counter = b.dcounter;
-  a.nitems = (int64_t)b.hitems.size();
-  a.complex3m = b.h->complex_mult == 3;
-  size_t lds = b.lds;
-  a.item_off = (int32_t)batch_item_offset(is_double(b.dtype), is_complex(b.dtype), lds);
-  launch_batch(is_double(b.dtype), is_complex(b.dtype), a, b.workgroups, lds, s);
+  const int64_t nmain = (int64_t)b.hitems.size() - b.nshallow;
+  if (nmain > 0) {
+    BatchArgs a;
+    a.pairs = b.dpairs;
+    a.items = b.ditems;
+    a.counter = b.dcounter;
+    a.nitems = nmain;
+    a.complex3m = b.h->complex_mult == 3;
+    size_t lds = b.lds;
+    a.item_off = (int32_t)batch_item_offset(is_double(b.dtype), is_complex(b.dtype), lds);
+    launch_batch(is_double(b.dtype), is_complex(b.dtype), a, b.workgroups, lds, s);
+  }
+  if (b.nshallow > 0) {
+    ShallowArgs a;
+    a.pairs = b.dpairs;
+    a.items = b.ditems + nmain;
+    a.nitems = b.nshallow;
+    launch_shallow(is_double(b.dtype), is_complex(b.dtype), a, s);
+  }
 }
 
 void batch_run(Batch& b, int mode) {
@@ -443,6 +460,7 @@ int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t*
     if (!b || !info) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null batch or info");
     info->pairs = (int64_t)b->cs.size();
     info->items = (int64_t)b->hitems.size();
+    info->shallow_items = b->nshallow;
     info->workgroups = b->workgroups;
     info->graph = b->exec ? 1 : 0;
     info->flops = b->flops;

@@ -51,6 +51,7 @@ class BatchInfo(ctypes.Structure):
         ("flops", ctypes.c_double),
         ("bytes", ctypes.c_double),
         ("table_bytes", ctypes.c_int64),
+        ("shallow_items", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:

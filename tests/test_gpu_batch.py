@@ -55,7 +55,7 @@ def test_batch_matches_oracle(T, tol):
     pairs, refs = make_pairs(rng, T)
     batch = ContractionBatch(pairs)
     info = batch.info()
-    assert info["pairs"] == len(pairs) and info["items"] > 0
+    assert info["pairs"] == len(pairs) and info["items"] > info["shallow_items"] > 0  # both kernels run
     batch.run("persistent")
     check(pairs, refs, tol)
     assert batch.info()["graph"] == 1
@@ -163,4 +163,42 @@ def test_largest_pairs_run_as_single_launches():
     torch.cuda.synchronize()
     for (c, a, b), e in zip(pairs[-2:], eager[-2:]):
         assert torch.equal(c.parent(), e)
+    batch.close()
+
+
+SHALLOW = [
+    (["i", "j"], (67, 16), ["j", "k"], (16, 93), ["k", "i"]),  # K = 16, ragged tiles, C along N
+    (["i", "j"], (67, 17), ["j", "k"], (17, 93), ["i", "k"]),  # K = 17: the MFMA kernel
+    (["a", "b"], (130, 9), ["b", "c"], (9, 70), ["a", "c"]),  # two K passes, C along M
+    (["j", "i"], (12, 100), ["k", "j"], (40, 12), ["i", "k"]),  # A contiguous along K
+    (["i", "j", "h"], (40, 3, 5), ["j", "k", "h"], (3, 50, 5), ["h", "k", "i"]),  # hyper modes
+    (["i"], (300,), ["k"], (200,), ["k", "i"]),  # outer product (K = 1)
+    (["i", "j"], (5, 2), ["j", "k"], (2, 3), ["i", "k"]),  # one partial tile
+]
+
+
+@pytest.mark.parametrize("T, tol", [(np.complex128, 1e-12), (np.float64, 1e-12), (np.complex64, 2e-5)])
+@pytest.mark.parametrize("tile", ["64x64", "32x128", "128x32"])
+def test_shallow_pairs(T, tol, tile):
+    """Pairs with K <= 16 run in the vector-FMA kernel (one 512-thread workgroup
+    per output tile); ragged tiles, both C orientations, K-major operands,
+    hyper modes and two K passes match the oracle."""
+    rng = np.random.default_rng(17)
+    pairs, refs = [], []
+    for ia, sa, ib, sb, ic in SHALLOW:
+        A, B = randn(rng, sa, T), randn(rng, sb, T)
+        a, b = dev_tensor(A, ia), dev_tensor(B, ib)
+        size = dict(zip(ia, sa)) | dict(zip(ib, sb))
+        c = muscle.Tensor(muscle.empty_col_major([size[x] for x in ic], a.dtype, a.device), ic)
+        pairs.append((c, a, b))
+        refs.append(bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib))
+    batch = ContractionBatch(pairs, tiles=[tile] * len(pairs))
+    info = batch.info()
+    assert 0 < info["shallow_items"] < info["items"]
+    batch.run()
+    check(pairs, refs, tol)
+    for c, a, b in pairs:
+        c.parent().fill_(float("nan"))
+    batch.run("persistent_nograph")
+    check(pairs, refs, tol)
     batch.close()
