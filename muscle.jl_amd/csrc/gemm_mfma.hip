@@ -142,8 +142,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   };
   Cplx<T> ra[NE], rb[NE];
   // a tile whose rows and columns all lie inside M and N needs no per-element
-  // checks on the K-tiles that lie inside its K range (all but the last)
-  const bool full_mn = m0 + BM <= p.M && n0 + BN <= p.N;
+  // checks on the K-tiles that lie inside its K range (all but the last);
+  // FP64 data on FP32 MFMA keeps the checked form, which measured faster there
+  const bool full_mn = sizeof(T) == sizeof(S) && m0 + BM <= p.M && n0 + BN <= p.N;
   auto gather = [&](int kt) {
     if (full_mn && kbeg + (kt + 1) * BK <= kend) {
 #pragma unroll
