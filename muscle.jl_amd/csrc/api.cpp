@@ -123,6 +123,27 @@ void upload_tables(Plan& p, hipStream_t s) {
   MH_HIP_CHECK(hipMemcpyAsync(p.dtab, p.htab.data(), bytes, hipMemcpyHostToDevice, s));
 }
 
+// Split-K for tile GEMMs with too few output tiles to fill 256 CUs: slices
+// of at least 64 along K, partial sums bounded to 256 MiB.  Plans of other
+// kinds keep one slice.
+void plan_gemm_ksplit(Plan& p) {
+  const ContractionDesc& d = p.d;
+  p.gemm_ksplit = 1;
+  p.gemm_kchunk = (int)std::min<int64_t>(d.K, INT32_MAX);
+  if (p.kind != Plan::GEMM) return;
+  const int64_t tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64) * d.B;
+  if (tiles < 512 && d.K >= 128) {
+    int64_t ks = std::min<int64_t>((1024 + tiles - 1) / tiles, d.K / 64);
+    const int64_t elem = (int64_t)dtype_size(d.dt[2]);
+    const int64_t cap = (int64_t(256) << 20) / std::max<int64_t>(1, d.M * d.N * d.B * elem);
+    ks = std::max<int64_t>(1, std::min<int64_t>({ks, 64, cap}));
+    int64_t kc = (d.K + ks - 1) / ks;
+    kc = (kc + 15) / 16 * 16;
+    p.gemm_kchunk = (int)kc;
+    p.gemm_ksplit = (int)((d.K + kc - 1) / kc);
+  }
+}
+
 // Builds offset tables and picks a kernel for a lowered contraction.
 std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int precision) {
   std::unique_ptr<Plan> p(new Plan());
@@ -229,23 +250,7 @@ std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int preci
   } else {
     p->kind = Plan::REDUCE;
   }
-  // split-K for tile GEMMs with too few output tiles to fill 256 CUs:
-  // slices of at least 64 along K, partial sums bounded to 256 MiB
-  p->gemm_ksplit = 1;
-  p->gemm_kchunk = (int)std::min<int64_t>(d.K, INT32_MAX);
-  if (p->kind == Plan::GEMM) {
-    const int64_t tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64) * d.B;
-    if (tiles < 512 && d.K >= 128) {
-      int64_t ks = std::min<int64_t>((1024 + tiles - 1) / tiles, d.K / 64);
-      const int64_t elem = (int64_t)dtype_size(d.dt[2]);
-      const int64_t cap = (int64_t(256) << 20) / std::max<int64_t>(1, d.M * d.N * d.B * elem);
-      ks = std::max<int64_t>(1, std::min<int64_t>({ks, 64, cap}));
-      int64_t kc = (d.K + ks - 1) / ks;
-      kc = (kc + 15) / 16 * 16;
-      p->gemm_kchunk = (int)kc;
-      p->gemm_ksplit = (int)((d.K + kc - 1) / kc);
-    }
-  }
+  plan_gemm_ksplit(*p);
   // reduction configuration (also the fallback of the other kinds)
   const int64_t outputs = d.M * d.N * d.B;
   int gshift = 0;

@@ -110,6 +110,8 @@ struct Handle {
 extern thread_local std::string g_last_error;
 
 std::unique_ptr<Plan> build_contraction_plan(const ContractionDesc& d, int precision);
+// Split-K slices of a tile-GEMM plan (one slice for plans of other kinds).
+void plan_gemm_ksplit(Plan& p);
 void upload_tables(Plan& p, hipStream_t s);
 // Enqueues one planned contraction on h.stream.  Its workspace is the
 // handle's (grown as needed), or `ws` of `ws_size` bytes when given (no

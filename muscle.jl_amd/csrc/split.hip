@@ -571,6 +571,7 @@ void prepare_split(Plan& p, double target) {
     p.kind = Plan::GEMM;
     p.fp32_math = true;
     p.err_bound = 1e-6;
+    plan_gemm_ksplit(p);
     return;
   }
   const bool ca = is_complex(d.dt[0]), cb = is_complex(d.dt[1]);
