@@ -289,13 +289,17 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
 }
 
 // C[bz, m, n] = sum over slices s of partial[s][bz][m][n], in slice order.
+// Small (64-thread) workgroups spread the few outputs of a split-K GEMM over
+// many CUs, and the unrolled slice loop keeps 8 loads of each thread in flight.
+constexpr int NTF = 64;
 template <typename T, typename S, bool CC>
-__global__ __launch_bounds__(NT) void splitk_finish_kernel(GemmArgs p) {
+__global__ __launch_bounds__(NTF) void splitk_finish_kernel(GemmArgs p) {
   const int64_t mn = (int64_t)p.M * p.N;
   const int64_t total = mn * p.batch;
-  const int64_t o = (int64_t)blockIdx.x * NT + threadIdx.x;
+  const int64_t o = (int64_t)blockIdx.x * NTF + threadIdx.x;
   if (o >= total) return;
   T re = 0, im = 0;
+#pragma unroll 8
   for (int s = 0; s < p.ksplit; ++s) {
     const Cplx<T> v = load_elem<T, CC>(p.partial, (int64_t)s * total + o);
     re += v.re;
@@ -322,8 +326,8 @@ void launch_t(const GemmArgs& g, hipStream_t s) {
   }
   if (g.ksplit > 1) {
     const int64_t total = (int64_t)g.M * g.N * g.batch;
-    hipLaunchKernelGGL((splitk_finish_kernel<T, S, CA || CB>), dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0,
-                       s, g);
+    hipLaunchKernelGGL((splitk_finish_kernel<T, S, CA || CB>), dim3((unsigned)((total + NTF - 1) / NTF)), dim3(NTF),
+                       0, s, g);
   }
 }
 
