@@ -191,9 +191,25 @@ def check(status: int) -> None:
 class _Handles(threading.local):
     def __init__(self):
         self.by_device = {}
+        self.stream = {}  # device -> the stream last set on that handle
+        self.option = {}  # (device, option) -> the value last set on that handle
 
 
 _handles = _Handles()
+
+
+def set_stream(device: int, h, stream: int) -> None:
+    """Points the handle at `stream` (skipped when it already is)."""
+    if _handles.stream.get(device) != stream:
+        check(lib.muscle_hip_set_stream(h, ctypes.c_void_p(stream)))
+        _handles.stream[device] = stream
+
+
+def set_option(device: int, h, option: int, value: int) -> None:
+    """muscle_hip_set_option (skipped when the handle already holds the value)."""
+    if _handles.option.get((device, option)) != value:
+        check(lib.muscle_hip_set_option(h, option, value))
+        _handles.option[(device, option)] = value
 
 
 def handle(device: int) -> ctypes.c_void_p:
@@ -214,14 +230,13 @@ class Desc:
 
     def __init__(self, dtype: int, data_ptr: int, modes, extents, strides):
         r = len(modes)
-        m = (ctypes.c_int32 * max(r, 1))(*modes)
-        e = (ctypes.c_int64 * max(r, 1))(*extents)
-        s = (ctypes.c_int64 * max(r, 1))(*strides)
+        n = max(r, 1)
+        m = (ctypes.c_int32 * n)(*modes)
+        e = (ctypes.c_int64 * n)(*extents)
+        s = (ctypes.c_int64 * n)(*strides)
         self._keep = (m, e, s)
-        self.desc = TensorDesc(dtype, r, ctypes.c_void_p(data_ptr),
-                               ctypes.cast(m, ctypes.POINTER(ctypes.c_int32)),
-                               ctypes.cast(e, ctypes.POINTER(ctypes.c_int64)),
-                               ctypes.cast(s, ctypes.POINTER(ctypes.c_int64)))
+        # ctypes turns each array into the pointer field's type on assignment
+        self.desc = TensorDesc(dtype, r, data_ptr, m, e, s)
 
     @property
     def ref(self):

@@ -44,7 +44,7 @@ class ContractionBatch:
         if tiles is not None:
             tl = (ctypes.c_int32 * n)(*[TILES[t] if isinstance(t, str) else int(t) for t in tiles])
         self._h = _handle_for(self.device)
-        _lib.check(_lib.lib.muscle_hip_set_option(self._h, 0, int(complex_mult)))
+        _lib.set_option(self.device, self._h, 0, int(complex_mult))
         self._b = ctypes.c_void_p()
         _lib.check(_lib.lib.muscle_hip_batch_create(self._h, n, cs, as_, bs, tl, ctypes.byref(self._b)))
 

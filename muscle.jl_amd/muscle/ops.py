@@ -83,22 +83,29 @@ def _device_of(*ts: Tensor) -> int:
 def _descs(*ts: Tensor):
     labels = {}
     for t in ts:
-        for i in t.inds():
+        for i in t._inds:
             labels.setdefault(i, len(labels) + 1)
     out = []
     for t in ts:
-        d = t.parent()
+        d = t._data
         code = _lib.DTYPE_CODE.get(d.dtype)
         if code is None:
             raise ArgumentError(f"unsupported eltype {d.dtype}")
-        out.append(_lib.Desc(code, d.data_ptr(), [labels[i] for i in t.inds()], list(d.shape), list(d.stride())))
+        out.append(_lib.Desc(code, d.data_ptr(), [labels[i] for i in t._inds], d.shape, d.stride()))
     return out
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _handle_for(dev: int):
+    """The thread's handle for `dev` (an index or a torch.device), pointed at
+    torch's current stream there."""
+    if isinstance(dev, torch.device):
+        dev = dev.index if dev.index is not None else torch.cuda.current_device()
     h = _lib.handle(dev)
-    stream = torch.cuda.current_stream(dev).cuda_stream
-    _lib.check(_lib.lib.muscle_hip_set_stream(h, ctypes.c_void_p(stream)))
+    stream = _raw_stream(dev) if _raw_stream is not None else torch.cuda.current_stream(dev).cuda_stream
+    _lib.set_stream(dev, h, stream)
     return h
 
 
@@ -115,7 +122,7 @@ def hip_binary_einsum_(backend: BackendHIP, c: Tensor, a: Tensor, b: Tensor) -> 
     prec = _lib.PRECISION.get(backend.precision)
     if prec is None:
         raise ArgumentError(f"unknown precision {backend.precision!r}")
-    _lib.check(_lib.lib.muscle_hip_set_option(h, 0, int(backend.complex_mult)))
+    _lib.set_option(dev, h, 0, int(backend.complex_mult))
     _lib.check(_lib.lib.muscle_hip_binary_einsum(h, dc.ref, da.ref, db.ref, prec, float(backend.target_rel_error),
                                                  ctypes.byref(info)))
     _record(info)
