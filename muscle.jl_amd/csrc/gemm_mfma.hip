@@ -69,8 +69,9 @@ constexpr int LDP = 16;  // LDS row padding (elements)
 // for B (else along N), or -1 to read both from the arguments.  Fixed at
 // compile time, the K-offset (or row-offset) registers that every element of
 // a thread shares collapse into one (FP64: 233 -> 227 VGPRs, config 2 from
-// 68 to 70.5 TF); the FP32 kernels keep the run-time form, which measured
-// faster for them.
+// 68 to 70.5 TF; ComplexF32: 160 -> 142..154 VGPRs, config 2 from 128 to
+// 131 TF and the dims-2 4096-cube from 119 to 126 TF).  The FP32 rung of the
+// ladder keeps the run-time form, which measured the same there.
 template <typename T, typename S, bool CA, bool CB, bool G3, int BK, int MAJ>
 __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
   const bool AK = MAJ < 0 ? p.a_kmajor != 0 : (MAJ & 2) != 0;
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(NTF) void splitk_finish_kernel(GemmArgs p) {
 template <typename T, typename S, bool CA, bool CB, bool G3>
 void launch_t(const GemmArgs& g, hipStream_t s) {
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.batch, (unsigned)g.ksplit);
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) != sizeof(S)) {
     hipLaunchKernelGGL((gemm_tile_kernel<T, S, CA, CB, G3, 16, -1>), grid, dim3(NT), 0, s, g);
   } else {
     switch ((g.a_kmajor ? 2 : 0) | (g.b_kmajor ? 1 : 0)) {
