@@ -1,4 +1,5 @@
-// Persistent batched contraction kernel.
+// Batched contraction kernels: the persistent MFMA kernel (pairs with K > 16)
+// and the shallow vector-FMA kernel (pairs with K <= 16, further below).
 //
 // Thousands of uneven pairwise contractions (the pairs of a contraction
 // tree) run in one launch: a fixed grid of workgroups (2 per CU) pulls
