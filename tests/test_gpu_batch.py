@@ -177,7 +177,8 @@ SHALLOW = [
 ]
 
 
-@pytest.mark.parametrize("T, tol", [(np.complex128, 1e-12), (np.float64, 1e-12), (np.complex64, 2e-5)])
+@pytest.mark.parametrize("T, tol", [(np.complex128, 1e-12), (np.float64, 1e-12), (np.complex64, 2e-5),
+                                    (np.float32, 2e-5)])
 @pytest.mark.parametrize("tile", ["64x64", "32x128", "128x32"])
 def test_shallow_pairs(T, tol, tile):
     """Pairs with K <= 16 run in the vector-FMA kernel (one 512-thread workgroup
