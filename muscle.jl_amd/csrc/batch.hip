@@ -41,11 +41,46 @@ struct Mma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// The last slice of a split tile to arrive (the partial tiles hold pm x pn
+// elements per tile and slice, pm = min(M, BM), pn = min(N, BN)) sums the
+// slices in slice order and writes C, so the result does not depend on which
+// slice finishes last.  Every thread of the workgroup calls it after writing
+// its part of this slice's partial tile.
+template <typename T, bool CP, int BM, int BN>
+__device__ __forceinline__ void split_tile_fixup(const BatchPair& pr, int bz, int tm, int tn, int* s_flag) {
+  const int tid = threadIdx.x;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int pm = min(pr.M, BM), pn = min(pr.N, BN);
+  const int mr = min(BM, pr.M - m0), nr = min(BN, pr.N - n0);
+  const int64_t tile = (int64_t)bz * pr.tiles_m * pr.tiles_n + (int64_t)tm * pr.tiles_n + tn;
+  const int64_t pstride = (int64_t)pm * pn, pbase = tile * pr.ksplit * pstride;
+  __threadfence();  // this slice's partial is visible device-wide before it is counted
+  __syncthreads();
+  if (tid == 0) *s_flag = atomicAdd(pr.tilecnt + tile, 1) == pr.ksplit - 1;
+  __syncthreads();
+  const bool last = *s_flag != 0;
+  __syncthreads();  // every thread has read the flag before its word is reused
+  if (!last) return;
+  __threadfence();  // the other slices' partials are read after their counts
+  const int64_t offC = pr.bC[bz];
+  for (int e = tid; e < mr * nr; e += NT) {
+    const int m = e / nr, n = e - m * nr;
+    Cplx<T> acc = {T(0), T(0)};
+    for (int q = 0; q < pr.ksplit; ++q) {
+      const Cplx<T> v = load_elem<T, CP>(pr.partial, pbase + q * pstride + (int64_t)m * pn + n);
+      acc.re += v.re;
+      acc.im += v.im;
+    }
+    store_elem<T, CP>(pr.C, offC + pr.rowC[m0 + m] + pr.colC[n0 + n], acc);
+  }
+}
+
 // DB: double-buffered LDS (one barrier per K-tile, the gather of K-tile
 // t + 1 in flight while t is multiplied); the 64 x 64 shape uses it, the
 // skinny shapes keep one buffer so every shape fits the same LDS budget.
-template <typename T, bool CP, bool G3, int BM, int BN, int WM, int WN, bool DB>
-__device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, int tn, T* lds) {
+template <typename T, bool CP, bool G3, int BM, int BN, int WM, int WN, bool DB, bool SPLIT>
+__device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, int tn, int slice, T* lds,
+                                           int* s_flag) {
   using MM = Mma<T>;
   using acc_t = typename MM::acc_t;
   constexpr int PL = CP ? 2 : 1;
@@ -57,6 +92,8 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
   const int wm = wave / WN, wn = wave % WN;
   const int m0 = tm * BM, n0 = tn * BN;
   const int M = pr.M, N = pr.N, K = pr.K;
+  // this item's K range: the whole of K, or one slice of a split pair
+  const int kbeg = slice * pr.kchunk, kend = min(K, kbeg + pr.kchunk);
   const int64_t offA0 = pr.bA[bz], offB0 = pr.bB[bz], offC0 = pr.bC[bz];
 
   int a_mm[NEA], a_kk[NEA], b_nn[NEB], b_kk[NEB];
@@ -103,13 +140,13 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
   do {                                                                           \
     _Pragma("unroll") for (int i = 0; i < NEA; ++i) {                            \
       const int ka = (kt) * BK + a_kk[i];                                        \
-      const Cplx<T> v = load_elem<T, CP>(pr.A, a_row[i] + pr.kA[min(ka, K - 1)]); \
-      ra[i] = (a_ok[i] && ka < K) ? v : Cplx<T>{T(0), T(0)};                     \
+      const Cplx<T> v = load_elem<T, CP>(pr.A, a_row[i] + pr.kA[min(ka, kend - 1)]); \
+      ra[i] = (a_ok[i] && ka < kend) ? v : Cplx<T>{T(0), T(0)};                  \
     }                                                                            \
     _Pragma("unroll") for (int i = 0; i < NEB; ++i) {                            \
       const int kb = (kt) * BK + b_kk[i];                                        \
-      const Cplx<T> v = load_elem<T, CP>(pr.B, b_col[i] + pr.kB[min(kb, K - 1)]); \
-      rb[i] = (b_ok[i] && kb < K) ? v : Cplx<T>{T(0), T(0)};                     \
+      const Cplx<T> v = load_elem<T, CP>(pr.B, b_col[i] + pr.kB[min(kb, kend - 1)]); \
+      rb[i] = (b_ok[i] && kb < kend) ? v : Cplx<T>{T(0), T(0)};                  \
     }                                                                            \
   } while (0)
 #define MH_STASH(buf)                                                            \
@@ -160,25 +197,26 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
     }                                                                            \
   } while (0)
 
-  const int nk = (K + BK - 1) / BK;
-  if (nk > 0) MH_GATHER(0);
+  // K-tiles kt0 .. kt1 - 1 (kbeg is a multiple of BK)
+  const int kt0 = kbeg / BK, kt1 = (kend + BK - 1) / BK;
+  if (kt0 < kt1) MH_GATHER(kt0);
   if constexpr (DB) {
     __syncthreads();  // the previous tile's readers of buffer 0 are done
-    if (nk > 0) MH_STASH(0);
+    if (kt0 < kt1) MH_STASH(0);
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      const int cur = kt & 1;
-      if (kt + 1 < nk) MH_GATHER(kt + 1);
+    for (int kt = kt0; kt < kt1; ++kt) {
+      const int cur = (kt - kt0) & 1;
+      if (kt + 1 < kt1) MH_GATHER(kt + 1);
       MH_COMPUTE(cur);
-      if (kt + 1 < nk) MH_STASH(cur ^ 1);
+      if (kt + 1 < kt1) MH_STASH(cur ^ 1);
       __syncthreads();
     }
   } else {
-    for (int kt = 0; kt < nk; ++kt) {
+    for (int kt = kt0; kt < kt1; ++kt) {
       __syncthreads();  // earlier readers of the LDS tile are done
       MH_STASH(0);
       __syncthreads();
-      if (kt + 1 < nk) MH_GATHER(kt + 1);
+      if (kt + 1 < kt1) MH_GATHER(kt + 1);
       MH_COMPUTE(0);
     }
   }
@@ -186,19 +224,27 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
 #undef MH_STASH
 #undef MH_COMPUTE
   // C's offsets for this thread's rows and columns, read before any store
-  // (a table load after a store to C could not be hoisted: they may alias)
+  // (a table load after a store to C could not be hoisted: they may alias).
+  // One slice of a split tile stores to its partial tile instead (pm x pn,
+  // row-major), with the same loop.
+  constexpr bool split = SPLIT;
+  void* const dst = split ? pr.partial : pr.C;
+  const int pn = min(N, BN);
+  const int64_t pb = split ? (((int64_t)bz * pr.tiles_m * pr.tiles_n + (int64_t)tm * pr.tiles_n + tn) * pr.ksplit +
+                              slice) * ((int64_t)min(M, BM) * pn)
+                           : 0;
   int64_t crow[TM][4], ccol[TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = m0 + wm * (BM / WM) + i * 16 + MM::row(lane, r);
-      crow[i][r] = m < M ? pr.rowC[m] : 0;
+      const int ml = wm * (BM / WM) + i * 16 + MM::row(lane, r), m = m0 + ml;
+      crow[i][r] = m < M ? (split ? pb + (int64_t)ml * pn : pr.rowC[m]) : 0;
     }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
-    ccol[j] = n < N ? offC0 + pr.colC[n] : 0;
+    const int nl = wn * (BN / WN) + j * 16 + (lane & 15), n = n0 + nl;
+    ccol[j] = n < N ? (split ? (int64_t)nl : offC0 + pr.colC[n]) : 0;
   }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -219,12 +265,19 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
           re = acc0[i][j][r];
           im = acc1[i][j][r];
         }
-        store_elem<T, CP>(pr.C, cn + crow[i][r], Cplx<T>{re, im});
+        store_elem<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});
       }
     }
+  if constexpr (split) split_tile_fixup<T, CP, BM, BN>(pr, bz, tm, tn, s_flag);
 }
 
-__global__ void reset_counter_kernel(int32_t* c) { *c = 0; }
+// Clears the work counter and the split tiles' arrival counters (a kernel,
+// not a memset, so that it can be graph captured).
+__global__ void reset_counter_kernel(int32_t* c, int32_t* tilecnt, int64_t n) {
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 == 0) *c = 0;
+  for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) tilecnt[i] = 0;
+}
 
 template <typename T, bool CP, bool G3>
 __global__ __launch_bounds__(NT, kBatchWgPerCu) void batch_kernel(BatchArgs a) {
@@ -242,13 +295,40 @@ __global__ __launch_bounds__(NT, kBatchWgPerCu) void batch_kernel(BatchArgs a) {
     const int2 w = a.items[it];
     const BatchPair& pr = a.pairs[w.x];
     const int per_batch = pr.tiles_m * pr.tiles_n;
-    const int bz = w.y / per_batch;
-    const int r = w.y - bz * per_batch;
+    // w.y = tile * ksplit + slice
+    const int t = w.y / pr.ksplit, slice = w.y - t * pr.ksplit;
+    const int bz = t / per_batch;
+    const int r = t - bz * per_batch;
     const int tm = r / pr.tiles_n, tn = r - tm * pr.tiles_n;
     switch (pr.tile) {
-      case 0: batch_tile<T, CP, G3, 64, 64, 2, 2, true>(pr, bz, tm, tn, lds); break;
-      case 1: batch_tile<T, CP, G3, 32, 128, 1, 4, false>(pr, bz, tm, tn, lds); break;
-      default: batch_tile<T, CP, G3, 128, 32, 4, 1, false>(pr, bz, tm, tn, lds); break;
+      case 0: batch_tile<T, CP, G3, 64, 64, 2, 2, true, false>(pr, bz, tm, tn, slice, lds, s_item); break;
+      case 1: batch_tile<T, CP, G3, 32, 128, 1, 4, false, false>(pr, bz, tm, tn, slice, lds, s_item); break;
+      default: batch_tile<T, CP, G3, 128, 32, 4, 1, false, false>(pr, bz, tm, tn, slice, lds, s_item); break;
+    }
+  }
+}
+
+// K slices of the split pairs: one (tile, slice) item per workgroup pass,
+// in a kernel of its own so that the persistent kernel above keeps its
+// register allocation.  The partial-tile stores and the last arriver's sum
+// are in batch_tile's SPLIT form.
+template <typename T, bool CP, bool G3>
+__global__ __launch_bounds__(NT, kBatchWgPerCu) void split_kernel(BatchArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  int* s_flag = reinterpret_cast<int*>(smem_raw + a.item_off);
+  for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const int2 w = a.items[it];
+    const BatchPair& pr = a.pairs[w.x];
+    const int per_batch = pr.tiles_m * pr.tiles_n;
+    const int t = w.y / pr.ksplit, slice = w.y - t * pr.ksplit;
+    const int bz = t / per_batch;
+    const int r = t - bz * per_batch;
+    const int tm = r / pr.tiles_n, tn = r - tm * pr.tiles_n;
+    switch (pr.tile) {
+      case 0: batch_tile<T, CP, G3, 64, 64, 2, 2, true, true>(pr, bz, tm, tn, slice, lds, s_flag); break;
+      case 1: batch_tile<T, CP, G3, 32, 128, 1, 4, false, true>(pr, bz, tm, tn, slice, lds, s_flag); break;
+      default: batch_tile<T, CP, G3, 128, 32, 4, 1, false, true>(pr, bz, tm, tn, slice, lds, s_flag); break;
     }
   }
 }
@@ -403,9 +483,14 @@ __global__ __launch_bounds__(SNT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
 }
 
 template <typename T, bool CP, bool G3>
-void launch_t(const BatchArgs& a, int wg, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-  hipLaunchKernelGGL((batch_kernel<T, CP, G3>), dim3(wg), dim3(NT), lds, s, a);
+void launch_t(const BatchArgs& a, const BatchArgs& sp, int wg, size_t lds, hipStream_t s) {
+  const unsigned rg = (unsigned)std::max<int64_t>(1, std::min<int64_t>((a.ntilecnt + 255) / 256, 1024));
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(rg), dim3(256), 0, s, a.counter, a.tilecnt, a.ntilecnt);
+  if (a.nitems > 0) hipLaunchKernelGGL((batch_kernel<T, CP, G3>), dim3(wg), dim3(NT), lds, s, a);
+  if (sp.nitems > 0) {
+    const unsigned g = (unsigned)std::min<int64_t>(sp.nitems, int64_t(wg) * 4);
+    hipLaunchKernelGGL((split_kernel<T, CP, G3>), dim3(g), dim3(NT), lds, s, sp);
+  }
 }
 
 }  // namespace
@@ -445,16 +530,16 @@ void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s) {
   }
 }
 
-void launch_batch(bool dbl, bool cplx, const BatchArgs& a, int wg, size_t lds, hipStream_t s) {
+void launch_batch(bool dbl, bool cplx, const BatchArgs& a, const BatchArgs& sp, int wg, size_t lds, hipStream_t s) {
   const bool g3 = cplx && a.complex3m;
   if (dbl) {
-    if (g3) launch_t<double, true, true>(a, wg, lds, s);
-    else if (cplx) launch_t<double, true, false>(a, wg, lds, s);
-    else launch_t<double, false, false>(a, wg, lds, s);
+    if (g3) launch_t<double, true, true>(a, sp, wg, lds, s);
+    else if (cplx) launch_t<double, true, false>(a, sp, wg, lds, s);
+    else launch_t<double, false, false>(a, sp, wg, lds, s);
   } else {
-    if (g3) launch_t<float, true, true>(a, wg, lds, s);
-    else if (cplx) launch_t<float, true, false>(a, wg, lds, s);
-    else launch_t<float, false, false>(a, wg, lds, s);
+    if (g3) launch_t<float, true, true>(a, sp, wg, lds, s);
+    else if (cplx) launch_t<float, true, false>(a, sp, wg, lds, s);
+    else launch_t<float, false, false>(a, sp, wg, lds, s);
   }
 }
 

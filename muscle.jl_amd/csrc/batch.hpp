@@ -20,6 +20,11 @@ struct BatchPair {
   int32_t M, N, K, NB;  // fused extents and batch (hyper) count
   int32_t tiles_m, tiles_n, tile, a_kmajor, b_kmajor;
   int32_t c_mfast;  // C's contiguous direction is M (else N)
+  // split K: ksplit slices of kchunk elements per output tile (1 and K when
+  // not split); the slices' partial tiles and the tiles' arrival counters
+  int32_t ksplit, kchunk;
+  void* partial;
+  int32_t* tilecnt;
   OffTab rowA, kA, kB, colB, rowC, colC, bA, bB, bC;
 };
 
@@ -34,6 +39,8 @@ struct BatchArgs {
   const BatchPair* pairs;
   const int2* items;  // (pair, tile within pair), largest pairs first
   int32_t* counter;   // work queue head
+  int32_t* tilecnt;   // arrival counters of the split pairs' tiles, cleared with the queue head
+  int64_t ntilecnt;
   int64_t nitems;
   int32_t complex3m;
   int32_t item_off;   // byte offset of the broadcast work item in dynamic LDS
@@ -52,10 +59,13 @@ struct ShallowArgs {
 
 // Launches the shallow kernel over a.nitems tiles (nothing when there are none).
 void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s);
-// Launches the persistent kernel (grid = workgroups) preceded by a one-thread
-// kernel that resets the work counter (no memset nodes: the launch is graph
-// captured).  lds_bytes: dynamic LDS for the largest tile shape present.
-void launch_batch(bool dbl, bool cplx, const BatchArgs& a, int workgroups, size_t lds_bytes, hipStream_t s);
+// Launches the persistent kernel (grid = workgroups) over a's items, then the
+// split kernel over split's items (the K slices of split pairs), preceded by a
+// kernel that resets the work counter and the split tiles' arrival counters
+// (no memset nodes: the launch is graph captured).  lds_bytes: dynamic LDS
+// for the largest tile shape present.
+void launch_batch(bool dbl, bool cplx, const BatchArgs& a, const BatchArgs& split, int workgroups,
+                  size_t lds_bytes, hipStream_t s);
 // Dynamic LDS one workgroup needs for a tile shape.
 size_t batch_lds_bytes(bool dbl, bool cplx, int tile);
 // Where the work-item broadcast word goes in a launch's dynamic LDS of
@@ -72,14 +82,25 @@ struct Batch {
   std::vector<std::vector<int32_t>> modes;      // storage behind cs/as/bs
   std::vector<std::vector<int64_t>> ext, str;
   std::vector<BatchPair> hpairs;
-  std::vector<int2> hitems;  // persistent items (largest K first), then shallow items
-  int64_t nshallow = 0;       // the shallow items at the end of hitems
+  // persistent items (largest K first), then the split pairs' slice items,
+  // then the shallow items
+  std::vector<int2> hitems;
+  int64_t nshallow = 0;  // the shallow items at the end of hitems
+  int64_t nsplit = 0;    // the slice items before them
+  // split K (per pair of the work list, by position): slices, K per slice,
+  // offsets of the pair's partial tiles (bytes) and arrival counters
+  std::vector<int32_t> ksplit, kchunk;
+  std::vector<int64_t> partial_off, tilecnt_off;
+  int64_t ntilecnt = 0;
+  size_t partial_bytes = 0;
+  int64_t split_items = 0;
   std::vector<int64_t> htab;
   std::vector<int> pair_class;
   void* dmem = nullptr;  // pairs | items | counter | tables
   BatchPair* dpairs = nullptr;
   int2* ditems = nullptr;
   int32_t* dcounter = nullptr;
+  int32_t* dtilecnt = nullptr;
   int workgroups = 0;
   size_t lds = 0;
   double flops = 0, bytes = 0;
@@ -121,10 +142,14 @@ struct PlannedPair {
   bool solo = false;              // runs as a single-contraction launch, not in the work list
 };
 
-// Device layout of a planned batch: pairs | items | counter | tables.
+// Device layout of a planned batch: pairs | items | counter | tables, copied
+// from the host (the first `copy` bytes), then the split pairs' arrival
+// counters and partial tiles, device-only.
 struct BatchLayout {
-  size_t pairs, items, counter, tab, total;
+  size_t pairs, items, counter, tab, tilecnt, partial, copy, total;
 };
+// Split K only while the partial tiles of one launch stay under this size.
+constexpr size_t kSplitPartialBytes = size_t(32) << 20;
 
 void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
                 const muscle_hip_tensor_t* bs, const int32_t* tiles, bool keep_descs, Batch& B,

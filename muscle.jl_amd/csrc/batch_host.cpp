@@ -113,29 +113,76 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
                    [&](int64_t x, int64_t y) { return lw[(size_t)x]->cost > lw[(size_t)y]->cost; });
   bool used[MUSCLE_HIP_TILE_COUNT] = {false, false, false};
   const bool cp = is_complex(B.dtype);
-  std::vector<int2> shallow;
-  for (int64_t i : order) {
-    const PlannedPair& L = *lw[(size_t)i];
-    if (L.d.empty) continue;
-    B.flops += (cp ? 8.0 : 2.0) * (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B;
-    if (L.solo) continue;
-    const TileShape& ts = kTileShapes[L.tile];
-    const int64_t tm = (L.d.M + ts.bm - 1) / ts.bm, tn = (L.d.N + ts.bn - 1) / ts.bn;
-    const int64_t items = tm * tn * L.d.B;
-    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
-    const bool sh = L.d.K <= kShallowK;
-    std::vector<int2>& dst = sh ? shallow : B.hitems;
-    for (int64_t t = 0; t < items; ++t) dst.push_back(make_int2((int)i, (int)t));
-    if (!sh) used[L.tile] = true;
-  }
-  B.nshallow = (int64_t)shallow.size();
-  B.hitems.insert(B.hitems.end(), shallow.begin(), shallow.end());
   int cus = h.cus;
   if (cus <= 0) {
     MH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h.device));
     h.cus = cus;
   }
   B.workgroups = std::max(1, cus) * kBatchWgPerCu;
+  auto tiles_of = [&](const PlannedPair& L) {
+    const TileShape& ts = kTileShapes[L.tile];
+    return ((L.d.M + ts.bm - 1) / ts.bm) * ((L.d.N + ts.bn - 1) / ts.bn) * L.d.B;
+  };
+  // Split K where a few long tiles would set the launch's length: a tile of
+  // more than 16 K-tiles and more than 4x the K-tiles per workgroup of the
+  // whole work list gets slices of about twice that share (at most 16), as
+  // long as the launch's partial tiles stay under kSplitPartialBytes.
+  double work = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const PlannedPair& L = *lw[(size_t)i];
+    if (L.d.empty || L.solo || L.d.K <= kShallowK) continue;
+    work += (double)tiles_of(L) * (double)((L.d.K + 15) / 16);
+  }
+  const double share = work / B.workgroups;
+  B.ksplit.assign((size_t)n, 1);
+  B.kchunk.assign((size_t)n, 0);
+  B.partial_off.assign((size_t)n, 0);
+  B.tilecnt_off.assign((size_t)n, 0);
+  B.ntilecnt = 0;
+  B.partial_bytes = 0;
+  B.split_items = 0;
+  const size_t es = dtype_size(B.dtype);
+  std::vector<int2> shallow, sliced;
+  for (int64_t i : order) {
+    const PlannedPair& L = *lw[(size_t)i];
+    if (L.d.empty) continue;
+    B.flops += (cp ? 8.0 : 2.0) * (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B;
+    if (L.solo) continue;
+    const TileShape& ts = kTileShapes[L.tile];
+    const int64_t items = tiles_of(L);
+    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
+    const bool sh = L.d.K <= kShallowK;
+    int64_t ks = 1, kc = L.d.K;
+    const int64_t kt = (L.d.K + 15) / 16;
+    if (!sh && kt > 16 && (double)kt > 4 * share) {
+      const int64_t slice_kt = std::max<int64_t>(4, (int64_t)(2 * share + 0.5));
+      ks = std::min<int64_t>(16, (kt + slice_kt - 1) / slice_kt);
+      kc = (kt + ks - 1) / ks * 16;
+      ks = (L.d.K + kc - 1) / kc;
+      const size_t pbytes = (size_t)items * (size_t)ks * (size_t)std::min<int64_t>(L.d.M, ts.bm) *
+                            (size_t)std::min<int64_t>(L.d.N, ts.bn) * es;
+      if (ks < 2 || (size_t)items * (size_t)ks >= (size_t(1) << 31) ||
+          B.partial_bytes + pbytes > kSplitPartialBytes) {
+        ks = 1;
+        kc = L.d.K;
+      } else {
+        B.partial_off[(size_t)i] = (int64_t)B.partial_bytes;
+        B.partial_bytes += (pbytes + 255) & ~size_t(255);
+        B.tilecnt_off[(size_t)i] = B.ntilecnt;
+        B.ntilecnt += items;
+        B.split_items += items * ks;
+      }
+    }
+    B.ksplit[(size_t)i] = (int32_t)ks;
+    B.kchunk[(size_t)i] = (int32_t)kc;
+    std::vector<int2>& dst = sh ? shallow : ks > 1 ? sliced : B.hitems;
+    for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
+    if (!sh) used[L.tile] = true;
+  }
+  B.nsplit = (int64_t)sliced.size();
+  B.nshallow = (int64_t)shallow.size();
+  B.hitems.insert(B.hitems.end(), sliced.begin(), sliced.end());
+  B.hitems.insert(B.hitems.end(), shallow.begin(), shallow.end());
   const bool dbl = is_double(B.dtype);
   for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
     if (used[t]) B.lds = std::max(B.lds, batch_lds_bytes(dbl, cp, t));
@@ -195,7 +242,10 @@ BatchLayout batch_layout(const Batch& B, int64_t n) {
   l.items = al(sizeof(BatchPair) * (size_t)std::max<int64_t>(n, 1));
   l.counter = l.items + al(sizeof(int2) * std::max<size_t>(B.hitems.size(), 1));
   l.tab = l.counter + 256;
-  l.total = l.tab + al(std::max<size_t>(B.htab.size(), 1) * sizeof(int64_t));
+  l.copy = l.tab + al(std::max<size_t>(B.htab.size(), 1) * sizeof(int64_t));
+  l.tilecnt = l.copy;
+  l.partial = l.tilecnt + al(sizeof(int32_t) * (size_t)std::max<int64_t>(B.ntilecnt, 1));
+  l.total = l.partial + B.partial_bytes;
   return l;
 }
 
@@ -207,6 +257,7 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
   B.dpairs = (BatchPair*)(base + l.pairs);
   B.ditems = (int2*)(base + l.items);
   B.dcounter = (int32_t*)(base + l.counter);
+  B.dtilecnt = (int32_t*)(base + l.tilecnt);
   const int64_t* dtab = (const int64_t*)(base + l.tab);
   B.hpairs.resize((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
@@ -228,6 +279,10 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
     P.a_kmajor = L.a_kmajor;
     P.b_kmajor = L.b_kmajor;
     P.c_mfast = L.c_mfast;
+    P.ksplit = B.ksplit.empty() ? 1 : B.ksplit[(size_t)i];
+    P.kchunk = B.kchunk.empty() || B.kchunk[(size_t)i] == 0 ? P.K : B.kchunk[(size_t)i];
+    P.partial = base + l.partial + (B.partial_off.empty() ? 0 : B.partial_off[(size_t)i]);
+    P.tilecnt = (int32_t*)(base + l.tilecnt) + (B.tilecnt_off.empty() ? 0 : B.tilecnt_off[(size_t)i]);
     OffTab* tabs[9] = {&P.rowA, &P.colB, &P.kA, &P.kB, &P.rowC, &P.colC, &P.bA, &P.bB, &P.bC};
     // tables were appended in the order rowA, kA, kB, colB, rowC, colC, bA, bB, bC
     const int map[9] = {0, 3, 1, 2, 4, 5, 6, 7, 8};
@@ -240,7 +295,7 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
     std::memcpy(staging + l.pairs, B.hpairs.data(), np);
     if (ni) std::memcpy(staging + l.items, B.hitems.data(), ni);
     if (nt) std::memcpy(staging + l.tab, B.htab.data(), nt);
-    MH_HIP_CHECK(hipMemcpyAsync(base, staging, l.total, hipMemcpyHostToDevice, s));
+    MH_HIP_CHECK(hipMemcpyAsync(base, staging, l.copy, hipMemcpyHostToDevice, s));
   } else {
     if (np) MH_HIP_CHECK(hipMemcpy(base + l.pairs, B.hpairs.data(), np, hipMemcpyHostToDevice));
     if (ni) MH_HIP_CHECK(hipMemcpy(base + l.items, B.hitems.data(), ni, hipMemcpyHostToDevice));
@@ -284,22 +339,27 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
 }
 
 void batch_launch(Batch& b, hipStream_t s) {
-  const int64_t nmain = (int64_t)b.hitems.size() - b.nshallow;
-  if (nmain > 0) {
+  const int64_t nmain = (int64_t)b.hitems.size() - b.nsplit - b.nshallow;
+  if (nmain > 0 || b.nsplit > 0) {
     BatchArgs a;
     a.pairs = b.dpairs;
     a.items = b.ditems;
     a.counter = b.dcounter;
+    a.tilecnt = b.dtilecnt;
+    a.ntilecnt = b.ntilecnt;
     a.nitems = nmain;
     a.complex3m = b.h->complex_mult == 3;
     size_t lds = b.lds;
     a.item_off = (int32_t)batch_item_offset(is_double(b.dtype), is_complex(b.dtype), lds);
-    launch_batch(is_double(b.dtype), is_complex(b.dtype), a, b.workgroups, lds, s);
+    BatchArgs sp = a;
+    sp.items = b.ditems + nmain;
+    sp.nitems = b.nsplit;
+    launch_batch(is_double(b.dtype), is_complex(b.dtype), a, sp, b.workgroups, lds, s);
   }
   if (b.nshallow > 0) {
     ShallowArgs a;
     a.pairs = b.dpairs;
-    a.items = b.ditems + nmain;
+    a.items = b.ditems + nmain + b.nsplit;
     a.nitems = b.nshallow;
     launch_shallow(is_double(b.dtype), is_complex(b.dtype), a, s);
   }
@@ -461,6 +521,7 @@ int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t*
     info->pairs = (int64_t)b->cs.size();
     info->items = (int64_t)b->hitems.size();
     info->shallow_items = b->nshallow;
+    info->split_items = b->split_items;
     info->workgroups = b->workgroups;
     info->graph = b->exec ? 1 : 0;
     info->flops = b->flops;

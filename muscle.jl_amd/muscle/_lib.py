@@ -52,6 +52,7 @@ class BatchInfo(ctypes.Structure):
         ("bytes", ctypes.c_double),
         ("table_bytes", ctypes.c_int64),
         ("shallow_items", ctypes.c_int64),
+        ("split_items", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
