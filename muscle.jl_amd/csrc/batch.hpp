@@ -148,8 +148,9 @@ struct PlannedPair {
 struct BatchLayout {
   size_t pairs, items, counter, tab, tilecnt, partial, copy, total;
 };
-// Split K only while the partial tiles of one launch stay under this size.
-constexpr size_t kSplitPartialBytes = size_t(32) << 20;
+// A pair of at most kSplitMaxTiles output tiles and at least kSplitMinKTiles
+// K-tiles (16 K each) runs as K slices (split kernel).
+constexpr int64_t kSplitMaxTiles = 2, kSplitMinKTiles = 32;
 
 void plan_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
                 const muscle_hip_tensor_t* bs, const int32_t* tiles, bool keep_descs, Batch& B,

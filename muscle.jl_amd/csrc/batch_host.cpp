@@ -123,17 +123,6 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     const TileShape& ts = kTileShapes[L.tile];
     return ((L.d.M + ts.bm - 1) / ts.bm) * ((L.d.N + ts.bn - 1) / ts.bn) * L.d.B;
   };
-  // Split K where a few long tiles would set the launch's length: a tile of
-  // more than 16 K-tiles and more than 4x the K-tiles per workgroup of the
-  // whole work list gets slices of about twice that share (at most 16), as
-  // long as the launch's partial tiles stay under kSplitPartialBytes.
-  double work = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    const PlannedPair& L = *lw[(size_t)i];
-    if (L.d.empty || L.solo || L.d.K <= kShallowK) continue;
-    work += (double)tiles_of(L) * (double)((L.d.K + 15) / 16);
-  }
-  const double share = work / B.workgroups;
   B.ksplit.assign((size_t)n, 1);
   B.kchunk.assign((size_t)n, 0);
   B.partial_off.assign((size_t)n, 0);
@@ -152,26 +141,24 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     const int64_t items = tiles_of(L);
     if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
     const bool sh = L.d.K <= kShallowK;
+    // Split K for a pair of few tiles (at most kSplitMaxTiles) and a long K
+    // (at least kSplitMinKTiles K-tiles): slices of at least 8 K-tiles, at
+    // most 16 of them.  Such tiles would otherwise set a small launch's
+    // length.  The rule reads the pair alone, so a pair's result does not
+    // depend on what else is in the launch.
     int64_t ks = 1, kc = L.d.K;
     const int64_t kt = (L.d.K + 15) / 16;
-    if (!sh && kt > 16 && (double)kt > 4 * share) {
-      const int64_t slice_kt = std::max<int64_t>(4, (int64_t)(2 * share + 0.5));
-      ks = std::min<int64_t>(16, (kt + slice_kt - 1) / slice_kt);
+    if (!sh && kt >= kSplitMinKTiles && items <= kSplitMaxTiles) {
+      ks = std::min<int64_t>(16, kt / 8);
       kc = (kt + ks - 1) / ks * 16;
       ks = (L.d.K + kc - 1) / kc;
       const size_t pbytes = (size_t)items * (size_t)ks * (size_t)std::min<int64_t>(L.d.M, ts.bm) *
                             (size_t)std::min<int64_t>(L.d.N, ts.bn) * es;
-      if (ks < 2 || (size_t)items * (size_t)ks >= (size_t(1) << 31) ||
-          B.partial_bytes + pbytes > kSplitPartialBytes) {
-        ks = 1;
-        kc = L.d.K;
-      } else {
-        B.partial_off[(size_t)i] = (int64_t)B.partial_bytes;
-        B.partial_bytes += (pbytes + 255) & ~size_t(255);
-        B.tilecnt_off[(size_t)i] = B.ntilecnt;
-        B.ntilecnt += items;
-        B.split_items += items * ks;
-      }
+      B.partial_off[(size_t)i] = (int64_t)B.partial_bytes;
+      B.partial_bytes += (pbytes + 255) & ~size_t(255);
+      B.tilecnt_off[(size_t)i] = B.ntilecnt;
+      B.ntilecnt += items;
+      B.split_items += items * ks;
     }
     B.ksplit[(size_t)i] = (int32_t)ks;
     B.kchunk[(size_t)i] = (int32_t)kc;

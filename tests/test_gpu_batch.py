@@ -207,8 +207,9 @@ def test_shallow_pairs(T, tol, tile):
 
 SPLIT = [
     (["i", "j"], (8, 4096), ["j", "k"], (4096, 4), ["i", "k"]),  # tiny output, long K
-    (["i", "j", "h"], (70, 1500, 2), ["j", "k", "h"], (1500, 40, 2), ["h", "k", "i"]),  # ragged tiles, hyper mode
-    (["a", "b"], (130, 2000), ["b", "c"], (2000, 66), ["c", "a"]),  # several tiles per slice
+    (["i", "j", "h"], (70, 1500, 1), ["j", "k", "h"], (1500, 40, 1), ["h", "k", "i"]),  # ragged tiles
+    (["i", "j", "h"], (30, 700, 2), ["j", "k", "h"], (700, 20, 2), ["h", "k", "i"]),  # hyper mode
+    (["a", "b"], (130, 2000), ["b", "c"], (2000, 66), ["c", "a"]),  # more tiles: not split
     (["i", "j"], (6, 5), ["j", "k"], (5, 7), ["i", "k"]),  # shallow
     (["i", "j"], (40, 20), ["j", "k"], (20, 30), ["k", "i"]),  # short K, not split
 ]
@@ -218,9 +219,10 @@ SPLIT = [
                                     (np.float32, 2e-5)])
 @pytest.mark.parametrize("tile", ["64x64", "32x128", "128x32"])
 def test_split_k_pairs(T, tol, tile):
-    """In a small batch, the few long-K tiles are cut into K slices (their own
-    launch) whose partial tiles the last-arriving slice sums in slice order:
-    results match the oracle, and graph replays equal the eager launch bit for bit."""
+    """Pairs of at most 2 output tiles and at least 32 K-tiles are cut into K
+    slices (their own launch) whose partial tiles the last-arriving slice sums
+    in slice order: results match the oracle, graph replays equal the eager
+    launch bit for bit, and a pair's result is the same alone."""
     rng = np.random.default_rng(18)
     pairs, refs = [], []
     for ia, sa, ib, sb, ic in SPLIT:
@@ -244,3 +246,11 @@ def test_split_k_pairs(T, tol, tile):
         for (c, a, b), e in zip(pairs, eager):
             assert torch.equal(c.parent(), e), rep
     batch.close()
+    c, a, b = pairs[0]  # the split pair alone gives the same bits as in the batch
+    alone = ContractionBatch([(c, a, b)], tiles=[tile])
+    assert alone.info()["split_items"] > 0
+    c.parent().fill_(float("nan"))
+    alone.run()
+    torch.cuda.synchronize()
+    assert torch.equal(c.parent(), eager[0])
+    alone.close()
