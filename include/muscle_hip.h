@@ -240,7 +240,7 @@ typedef struct {
   double bytes;       /* sum of algorithmic bytes (every operand element once) */
   int64_t table_bytes;/* device bytes of offset tables and pair descriptors */
   int64_t shallow_items;/* of `items`, the tiles of pairs with K <= 16 (vector-FMA kernel) */
-  int64_t split_items;/* of `items`, K slices of long tiles split so they do not set the launch's length */
+  int64_t split_items;/* of `items`, K slices of pairs with <= 2 output tiles and K >= 512 */
 } muscle_hip_batch_info_t;
 
 /* Plans `n` independent binary contractions C_i = A_i * B_i, each with the
