@@ -94,6 +94,7 @@ struct Batch {
   int64_t ntilecnt = 0;
   size_t partial_bytes = 0;
   int64_t split_items = 0;
+  bool allow_split = true;  // off for the autotuner's timing batches
   std::vector<int64_t> htab;
   std::vector<int> pair_class;
   void* dmem = nullptr;  // pairs | items | counter | tables
@@ -172,7 +173,7 @@ BatchLayout batch_layout(const Batch& B, int64_t n);
 void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base, char* staging, hipStream_t s);
 void batch_launch(Batch& b, hipStream_t s);
 std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
-                                   const muscle_hip_tensor_t* bs, const int32_t* tiles);
+                                   const muscle_hip_tensor_t* bs, const int32_t* tiles, bool allow_split = true);
 void batch_run(Batch& b, int mode);
 void batch_autotune(Batch& b, int samples, int reps);
 int batch_tuning_class(int64_t m, int64_t n);

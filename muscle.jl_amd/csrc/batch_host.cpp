@@ -148,7 +148,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     // depend on what else is in the launch.
     int64_t ks = 1, kc = L.d.K;
     const int64_t kt = (L.d.K + 15) / 16;
-    if (!sh && kt >= kSplitMinKTiles && items <= kSplitMaxTiles) {
+    if (B.allow_split && !sh && kt >= kSplitMinKTiles && items <= kSplitMaxTiles) {
       ks = std::min<int64_t>(16, kt / 8);
       kc = (kt + ks - 1) / ks * 16;
       ks = (L.d.K + kc - 1) / kc;
@@ -291,8 +291,9 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
 }
 
 std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor_t* cs, const muscle_hip_tensor_t* as,
-                                   const muscle_hip_tensor_t* bs, const int32_t* tiles) {
+                                   const muscle_hip_tensor_t* bs, const int32_t* tiles, bool allow_split) {
   std::unique_ptr<Batch> B(new Batch());
+  B->allow_split = allow_split;
   std::vector<PlannedPair> lw;
   plan_batch(h, n, cs, as, bs, tiles, true, *B, lw);
   const BatchLayout l = batch_layout(*B, n);
@@ -434,7 +435,10 @@ void batch_autotune(Batch& b, int samples, int reps) {
     int best_t = default_tile(b.hpairs[(size_t)idx[0]].M, b.hpairs[(size_t)idx[0]].N);
     for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t) {
       std::vector<int32_t> tiles(idx.size(), t);
-      std::unique_ptr<Batch> sub = build_batch(h, (int64_t)idx.size(), cs.data(), as.data(), bs.data(), tiles.data());
+      // timed without K splits: a class's shape is chosen on its tiles alone,
+      // as a sample of a few pairs would split where the whole batch need not
+      std::unique_ptr<Batch> sub =
+          build_batch(h, (int64_t)idx.size(), cs.data(), as.data(), bs.data(), tiles.data(), false);
       batch_run(*sub, 2);  // warm-up
       MH_HIP_CHECK(hipEventRecord(e0, h.stream));
       for (int r = 0; r < reps; ++r) batch_run(*sub, 2);
