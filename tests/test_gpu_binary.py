@@ -264,3 +264,23 @@ def test_skinny_products(case, T):
     c, got = _run(A, ia, B, ib, out=ic)
     assert muscle.last_info()["kernel"] == 1
     assert rel_err(got, ref) <= TOL[T], rel_err(got, ref)
+
+
+def test_handle_follows_device_forms_and_streams():
+    """One handle per thread and device whichever way the device is named, and
+    calls follow torch's current stream (the handle's stream is set again
+    whenever it changes)."""
+    from muscle.ops import _handle_for
+    h = [_handle_for(0), _handle_for(torch.device("cuda", 0)), _handle_for(torch.device("cuda"))]
+    assert len({x.value for x in h}) == 1
+    rng = np.random.default_rng(31)
+    A, B = randn(rng, (96, 80)), randn(rng, (80, 72))
+    a, b = dev_tensor(A, ["i", "j"]), dev_tensor(B, ["j", "k"])
+    ref = A @ B
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()  # the inputs' copies are done before the side stream reads them
+    for stream in (side, torch.cuda.default_stream(), side):
+        with torch.cuda.stream(stream):
+            c = muscle.binary_einsum(a, b)
+            got = c.parent().cpu().numpy()  # ordered after the call on this stream
+        assert rel_err(got, ref) <= 1e-13
