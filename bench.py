@@ -186,7 +186,8 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
                for (c, a, b), x in zip(round_tensors[-1], last_c))
     out = {
         "dtype": dtype, "pairs": n, "rounds": len(rounds), "output_budget_gib": budget_gb,
-        "items": sum(i["items"] for i in infos), "workgroups": infos[0]["workgroups"],
+        "items": sum(i["items"] for i in infos), "shallow_items": sum(i["shallow_items"] for i in infos),
+        "split_items": sum(i["split_items"] for i in infos), "workgroups": infos[0]["workgroups"],
         "graph": all(bt.info()["graph"] for bt in batches),
         "persistent_ms": round(ms_p, 3), "persistent_pairs_per_s": round(n / (ms_p / 1e3), 1),
         "persistent_tflops": round(flops / (ms_p / 1e3) / 1e12, 3),
