@@ -254,3 +254,21 @@ def test_split_k_pairs(T, tol, tile):
     torch.cuda.synchronize()
     assert torch.equal(c.parent(), eager[0])
     alone.close()
+
+
+def test_batch_contracted_extent_zero():
+    """In a batch, a pair whose contracted extent is 0 writes zeros (shallow
+    kernel with no K pass) beside a normal pair."""
+    rng = np.random.default_rng(19)
+    A, B = randn(rng, (20, 30)), randn(rng, (30, 40))
+    pairs = [(muscle.Tensor(torch.full((5, 6), float("nan"), dtype=torch.complex128, device="cuda"), ["i", "k"]),
+              dev_tensor(np.zeros((5, 0), dtype=np.complex128), ["i", "j"]),
+              dev_tensor(np.zeros((0, 6), dtype=np.complex128), ["j", "k"])),
+             (muscle.Tensor(muscle.empty_col_major([20, 40], torch.complex128, "cuda"), ["i", "k"]),
+              dev_tensor(A, ["i", "j"]), dev_tensor(B, ["j", "k"]))]
+    batch = ContractionBatch(pairs)
+    batch.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(host(pairs[0][0]), np.zeros((5, 6)))
+    assert rel_err(host(pairs[1][0]), A @ B) <= 1e-12
+    batch.close()

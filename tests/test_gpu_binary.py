@@ -284,3 +284,14 @@ def test_handle_follows_device_forms_and_streams():
             c = muscle.binary_einsum(a, b)
             got = c.parent().cpu().numpy()  # ordered after the call on this stream
         assert rel_err(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.float32])
+def test_contracted_extent_zero(T):
+    """A contracted mode of extent 0 sums nothing: C is all zeros (C's memory
+    starts as NaN, so every element is written)."""
+    a = dev_tensor(np.zeros((3, 0), dtype=T), ["i", "j"])
+    b = dev_tensor(np.zeros((0, 4), dtype=T), ["j", "k"])
+    c = muscle.Tensor(torch.full((3, 4), float("nan"), dtype=a.dtype, device="cuda"), ["i", "k"])
+    muscle.binary_einsum_(c, a, b)
+    assert np.array_equal(host(c), np.zeros((3, 4), dtype=T))
