@@ -490,6 +490,9 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     return;
   }
   if (p.kind == Plan::GEMM) {
+    // the planner sends empty K (sums of nothing) to the reduction; the tile
+    // kernels read their first K-tile's offsets unconditionally
+    if (d.K < 1) fail(MUSCLE_HIP_INVALID_ARGUMENT, "internal: tile GEMM planned for an empty K");
     if (p.pack_a) {
       char* buf = base + w.pack_a_off;
       launch_pack_operand((int)es_a, PackOpArgs{A, buf, p.tab(TAB_ROWA), p.tab(TAB_KA), p.tab(TAB_BA), (int32_t)d.M,
