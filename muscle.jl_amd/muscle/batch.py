@@ -44,12 +44,15 @@ class ContractionBatch:
         if tiles is not None:
             tl = (ctypes.c_int32 * n)(*[TILES[t] if isinstance(t, str) else int(t) for t in tiles])
         self._h = _handle_for(self.device)
-        _lib.set_option(self.device, self._h, 0, int(complex_mult))
+        self._complex_mult = int(complex_mult)
+        _lib.set_option(self.device, self._h, 0, self._complex_mult)
         self._b = ctypes.c_void_p()
         _lib.check(_lib.lib.muscle_hip_batch_create(self._h, n, cs, as_, bs, tl, ctypes.byref(self._b)))
 
     def run(self, mode: str = "persistent") -> None:
         h = _handle_for(self.device)
+        # the batch's complex product form, whatever calls ran on the handle since
+        _lib.set_option(self.device, h, 0, self._complex_mult)
         _lib.check(_lib.lib.muscle_hip_batch_run(h, self._b, MODES[mode]))
 
     def info(self) -> dict:
