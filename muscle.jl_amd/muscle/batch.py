@@ -62,6 +62,7 @@ class ContractionBatch:
 
     def autotune(self, samples: int = 16, reps: int = 3) -> None:
         h = _handle_for(self.device)
+        _lib.set_option(self.device, h, 0, self._complex_mult)
         _lib.check(_lib.lib.muscle_hip_batch_autotune(h, self._b, int(samples), int(reps)))
 
     def close(self) -> None:
