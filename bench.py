@@ -505,8 +505,9 @@ def main() -> int:
                          "traffic to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; how the path shards over GPUs is "
-                                   "described in DESIGN.md, and no multi-GPU figure is produced"}))
+        print(json.dumps({"error": "this benchmark measures one GPU only: the contractions shard by output "
+                                   "blocks, batch slices, pairs or jobs with no collective, and no multi-GPU "
+                                   "figure is produced"}))
         return 2
     only = set(args.only.split(","))
 

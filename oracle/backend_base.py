@@ -169,8 +169,8 @@ def binary_einsum_general(inds_c, a: np.ndarray, inds_a, b: np.ndarray, inds_b) 
 
     Semantics of ext/MuscleCUDAExt.jl:22-41 / ext/MuscleOMEinsumExt.jl:40-59,
     lowered as ext/MuscleReactantExt.jl:94-122 does (batching dims =
-    a∩b∩c, contracting dims = (a∩b)∖c) and computed as BASELINE's hyperindex
-    baseline does: the BackendBase permute + gemm, looped over the batch.
+    a∩b∩c, contracting dims = (a∩b)∖c) and computed the BackendBase way
+    (permute + gemm), looped over the batch.
     Repeated modes inside an operand and modes of one operand absent from
     everything else are first reduced with unary_einsum_base (OMEinsum's
     unary pre-pass).
