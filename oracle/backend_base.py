@@ -165,7 +165,7 @@ def binary_einsum_base_inplace(inds_c, a, inds_a, b, inds_b) -> np.ndarray:
 
 # ---------------------------------------------------------------- hyperindex extension
 def binary_einsum_general(inds_c, a: np.ndarray, inds_a, b: np.ndarray, inds_b) -> np.ndarray:
-    """General binary contraction with batch (hyper) modes (S4).
+    """General binary contraction with batch (hyper) modes.
 
     Semantics of ext/MuscleCUDAExt.jl:22-41 / ext/MuscleOMEinsumExt.jl:40-59,
     lowered as ext/MuscleReactantExt.jl:94-122 does (batching dims =
