@@ -201,6 +201,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gather(kt + 1);
     if (kt + 2 < nk) load_tables(kt + 2);
+    // FP64: the MFMA section issues at raised wave priority, so a wave that is
+    // multiplying is not held back by the other workgroup's waves issuing
+    // their gathers and LDS stores (config 2: 71.1 -> 76.2 TF)
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
@@ -242,6 +246,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
           }
       }
     }
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(0);
     if (kt + 1 < nk) stash(cur ^ 1);
     __syncthreads();
   }
