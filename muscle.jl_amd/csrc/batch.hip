@@ -207,7 +207,10 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
     for (int kt = kt0; kt < kt1; ++kt) {
       const int cur = (kt - kt0) & 1;
       if (kt + 1 < kt1) MH_GATHER(kt + 1);
+      // FP64: the MFMA section at raised wave priority, as in the plain tiles
+      if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(1);
       MH_COMPUTE(cur);
+      if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(0);
       if (kt + 1 < kt1) MH_STASH(cur ^ 1);
       __syncthreads();
     }
@@ -217,7 +220,9 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
       MH_STASH(0);
       __syncthreads();
       if (kt + 1 < kt1) MH_GATHER(kt + 1);
+      if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(1);
       MH_COMPUTE(0);
+      if constexpr (sizeof(T) == 8) __builtin_amdgcn_s_setprio(0);
     }
   }
 #undef MH_GATHER
