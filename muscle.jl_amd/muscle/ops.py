@@ -98,6 +98,32 @@ def _descs(*ts: Tensor):
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
+_slots = threading.local()
+
+
+def _call_descs(*ts: Tensor):
+    """Descriptors for one synchronous library call, filled into this thread's
+    reusable slots (valid until the thread's next call; use _descs for
+    descriptors that must outlive it)."""
+    pool = getattr(_slots, "pool", None)
+    if pool is None:
+        pool = _slots.pool = [_lib.DescSlot() for _ in range(4)]
+    if len(ts) > len(pool) or any(len(t._inds) > 64 for t in ts):
+        return _descs(*ts)
+    labels = {}
+    for t in ts:
+        for i in t._inds:
+            labels.setdefault(i, len(labels) + 1)
+    out = []
+    for t, slot in zip(ts, pool):
+        d = t._data
+        code = _lib.DTYPE_CODE.get(d.dtype)
+        if code is None:
+            raise ArgumentError(f"unsupported eltype {d.dtype}")
+        out.append(slot.fill(code, d.data_ptr(), [labels[i] for i in t._inds], d.shape, d.stride()))
+    return out
+
+
 def _handle_for(dev: int):
     """The thread's handle for `dev` (an index or a torch.device), pointed at
     torch's current stream there."""
@@ -117,7 +143,7 @@ def _record(info: _lib.Info) -> None:
 def hip_binary_einsum_(backend: BackendHIP, c: Tensor, a: Tensor, b: Tensor) -> Tensor:
     dev = _device_of(c, a, b)
     h = _handle_for(dev)
-    dc, da, db = _descs(c, a, b)
+    dc, da, db = _call_descs(c, a, b)
     info = _lib.Info()
     prec = _lib.PRECISION.get(backend.precision)
     if prec is None:
@@ -148,7 +174,7 @@ def hip_binary_einsum(backend: BackendHIP, inds_c, a: Tensor, b: Tensor) -> Tens
 def hip_hadamard_(backend: BackendHIP, c: Tensor, a: Tensor, b: Tensor) -> Tensor:
     dev = _device_of(c, a, b)
     h = _handle_for(dev)
-    dc, da, db = _descs(c, a, b)
+    dc, da, db = _call_descs(c, a, b)
     info = _lib.Info()
     _lib.check(_lib.lib.muscle_hip_hadamard(h, dc.ref, da.ref, db.ref, ctypes.byref(info)))
     _record(info)
@@ -166,7 +192,7 @@ def hip_hadamard(backend: BackendHIP, a: Tensor, b: Tensor) -> Tensor:
 def hip_unary_einsum_(backend: BackendHIP, y: Tensor, x: Tensor) -> Tensor:
     dev = _device_of(y, x)
     h = _handle_for(dev)
-    dy, dx = _descs(y, x)
+    dy, dx = _call_descs(y, x)
     info = _lib.Info()
     _lib.check(_lib.lib.muscle_hip_unary_einsum(h, dy.ref, dx.ref, ctypes.byref(info)))
     _record(info)

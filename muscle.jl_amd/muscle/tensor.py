@@ -17,18 +17,19 @@ from .errors import DimensionMismatch
 class Index:
     """``Index{T}`` holding an arbitrary hashable tag (src/Index.jl:3-5)."""
 
-    __slots__ = ("tag",)
+    __slots__ = ("tag", "_hash")
 
     def __init__(self, tag):
         if isinstance(tag, Index):
             tag = tag.tag
         self.tag = tag
+        self._hash = hash(("Index", tag))  # the index algebra hashes every label many times per call
 
     def __eq__(self, other):
-        return isinstance(other, Index) and self.tag == other.tag
+        return self is other or (isinstance(other, Index) and self._hash == other._hash and self.tag == other.tag)
 
     def __hash__(self):
-        return hash(("Index", self.tag))
+        return self._hash
 
     def __repr__(self):
         return f"index<{self.tag}>"
