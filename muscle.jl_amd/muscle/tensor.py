@@ -23,12 +23,25 @@ class Index:
         if isinstance(tag, Index):
             tag = tag.tag
         self.tag = tag
-        self._hash = hash(("Index", tag))  # the index algebra hashes every label many times per call
+        # the index algebra hashes every label many times per call; an
+        # unhashable tag makes an Index that compares but cannot be hashed
+        try:
+            self._hash = hash(("Index", tag))
+        except TypeError:
+            self._hash = None
 
     def __eq__(self, other):
-        return self is other or (isinstance(other, Index) and self._hash == other._hash and self.tag == other.tag)
+        if self is other:
+            return True
+        if not isinstance(other, Index):
+            return False
+        if self._hash is not None and other._hash is not None and self._hash != other._hash:
+            return False
+        return self.tag == other.tag
 
     def __hash__(self):
+        if self._hash is None:
+            raise TypeError(f"unhashable Index tag: {self.tag!r}")
         return self._hash
 
     def __repr__(self):
