@@ -180,3 +180,30 @@ def test_isapprox_is_permutation_aware():
     b = muscle.Tensor(x.T.contiguous(), ["j", "i"])
     assert a.isapprox(b)
     assert not a.isapprox(muscle.Tensor(x.T.contiguous() + 1, ["j", "i"]))
+
+
+def test_call_descriptor_slots_match_fresh_descriptors():
+    """The reusable per-thread descriptor slots of synchronous calls carry the
+    same dtype, rank, pointer, mode labels, extents and strides as freshly
+    built descriptors, also after a refill with other tensors."""
+    from muscle.ops import _call_descs, _descs
+    a = muscle.Tensor(torch.randn(4, 3, 5, dtype=torch.complex128).permute(2, 0, 1), ["k", "i", "j"])
+    b = muscle.Tensor(torch.randn(5, 6, dtype=torch.complex128), ["k", "l"])
+    c = muscle.Tensor(torch.empty(4, 3, 6, dtype=torch.complex128), ["i", "j", "l"])
+    x = muscle.Tensor(torch.randn(7, dtype=torch.float32), ["q"])
+    for ts in ((c, a, b), (x,), (c, a, b)):
+        for slot, fresh in zip(_call_descs(*ts), _descs(*ts)):
+            s, f = slot.desc, fresh.desc
+            assert (s.dtype, s.rank, s.data) == (f.dtype, f.rank, f.data)
+            for name in ("modes", "extents", "strides"):
+                assert [getattr(s, name)[i] for i in range(s.rank)] == [getattr(f, name)[i] for i in range(f.rank)]
+
+
+def test_index_hash_and_equality():
+    I = muscle.Index
+    assert I("a") == I("a") and I("a") != I("b") and hash(I(("site", 1))) == hash(I(("site", 1)))
+    assert {I("a"): 1}[I("a")] == 1 and I(I("a")) == I("a")
+    odd = I([1, 2])  # an unhashable tag: compares, cannot be hashed (as before)
+    assert odd == I([1, 2]) and odd != I("a")
+    with pytest.raises(TypeError):
+        hash(odd)
