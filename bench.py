@@ -93,8 +93,44 @@ def contraction_case(a, b, kw, backend, steps, warmup, settle_ms=0.0):
 
 
 # ------------------------------------------------------------------ CPU baseline
-def cpu_oracle(A, ia, B, ib, inds_c, threads: int, reps: int):
-    """The oracle's BackendBase restatement (permute + BLAS gemm + permute) on the host cores."""
+CPU_SHARE = 16  # host threads one GPU's process may use on the benchmark machine
+
+
+def cpu_facts() -> dict:
+    """The host CPU: model, logical CPUs (nproc), the CPUs this process may run on,
+    physical cores (machine and within that set), and the BLAS thread count the
+    baseline uses: the physical cores available, at most the per-GPU CPU share."""
+    import subprocess
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = sorted(os.sched_getaffinity(0))
+    cores_all, cores_allowed = set(), set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) if x else 0 for x in line.split(","))
+            cores_all.add((sock, core))
+            if cpu in allowed:
+                cores_allowed.add((sock, core))
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        cores_all = cores_allowed = set(range(len(allowed)))
+    phys = len(cores_allowed) or len(allowed)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(allowed),
+            "physical_cores": len(cores_all), "physical_cores_available": phys,
+            "cpu_share": CPU_SHARE, "blas_threads": max(1, min(phys, CPU_SHARE))}
+
+
+def cpu_oracle(A, ia, B, ib, inds_c, threads: int, reps: int, warm: bool = True):
+    """The oracle's BackendBase restatement (permute + BLAS gemm + permute) on the host cores:
+    median seconds of `reps` runs (after one untimed run when `warm`), the result, the BLAS."""
     import numpy as np
     from threadpoolctl import threadpool_info, threadpool_limits
 
@@ -102,8 +138,10 @@ def cpu_oracle(A, ia, B, ib, inds_c, threads: int, reps: int):
     general = any(i in ia and i in ib for i in inds_c)
     f = bb.binary_einsum_general if general else bb.binary_einsum_base
     times = []
+    ref = None
     with threadpool_limits(limits=threads):
-        ref = f(inds_c, A, ia, B, ib)
+        if warm:
+            ref = f(inds_c, A, ia, B, ib)
         for _ in range(reps):
             t0 = time.perf_counter()
             f(inds_c, A, ia, B, ib)
@@ -115,56 +153,27 @@ def cpu_oracle(A, ia, B, ib, inds_c, threads: int, reps: int):
 # ------------------------------------------------------------------ batched (config 4)
 def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, loop_steps: int, cpu_sample: int,
                   threads: int):
+    """Config 4: n uneven dims-2 pairs (muscle.workloads.circuit_batch_workload) as
+    persistent graph-replayed launches against the loop of plain calls.  The CPU
+    sample runs the oracle on the GPU's own inputs of `cpu_sample` seeded pairs and
+    checks the GPU's outputs of those pairs against it."""
     import numpy as np
     import torch
 
-    import muscle
     from muscle.batch import ContractionBatch
-    from muscle.workloads import circuit_pairs
-    dev = torch.device("cuda", 0)
-    tdt = torch.complex128 if dtype == "c64" else torch.complex64
+    from muscle.workloads import circuit_batch_workload
     esz = 16 if dtype == "c64" else 8
     t0 = time.perf_counter()
-    pairs = circuit_pairs(n, seed)
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    pool_n = 1 << 23  # inputs are views into one random pool (read-only, as tree tensors are reused)
-    pool = torch.randn(pool_n, dtype=tdt, device=dev, generator=g)
-    rng = np.random.default_rng(seed + 1)
-    budget = int(budget_gb * 2 ** 30) // esz
-    rounds, cur, used = [], [], 0
-    for p in pairs:
-        sz = 1 << len(p[2])
-        if cur and used + sz > budget:
-            rounds.append(cur)
-            cur, used = [], 0
-        cur.append(p)
-        used += sz
-    rounds.append(cur)
-    arena_n = max(sum(1 << len(p[2]) for p in r) for r in rounds)
-    arena = torch.empty(arena_n, dtype=tdt, device=dev)
-
-    def dims2(base, rank, off):
-        return base.as_strided((2,) * rank, tuple(1 << i for i in range(rank)), off)
-
-    round_tensors = []
-    for r in rounds:
-        ts, off = [], 0
-        for ia, ib, ic in r:
-            a = muscle.Tensor(dims2(pool, len(ia), int(rng.integers(0, pool_n - (1 << len(ia)) + 1))), ia)
-            b = muscle.Tensor(dims2(pool, len(ib), int(rng.integers(0, pool_n - (1 << len(ib)) + 1))), ib)
-            c = muscle.Tensor(dims2(arena, len(ic), off), ic)
-            off += 1 << len(ic)
-            ts.append((c, a, b))
-        round_tensors.append(ts)
-    log(f"batched {dtype}: {n} pairs in {len(rounds)} round(s) of <= {budget_gb:g} GiB outputs "
+    wl = circuit_batch_workload(dtype, n, seed, budget_gb)
+    pairs = wl.pairs
+    log(f"batched {dtype}: {n} pairs in {len(wl.rounds)} round(s) of <= {budget_gb:g} GiB outputs "
         f"(setup {time.perf_counter() - t0:.1f} s)")
-    probe = ContractionBatch(round_tensors[0])
+    probe = ContractionBatch(wl.rounds[0])
     t1 = time.perf_counter()
     probe.autotune(samples=16, reps=2)
     tune_s = time.perf_counter() - t1
     probe.close()
-    batches = [ContractionBatch(ts) for ts in round_tensors]
+    batches = [ContractionBatch(ts) for ts in wl.rounds]
     infos = [b.info() for b in batches]
     flops = sum(i["flops"] for i in infos)
     nbytes = sum(i["bytes"] for i in infos)
@@ -180,12 +189,13 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
         for bt in batches:
             bt.run(mode)
     ms_p, _ = timed(lambda: run_all("persistent"), steps, 1)
-    last_c = [c.parent().clone() for c, a, b in round_tensors[-1]]
+    last_c = [c.parent().clone() for c, a, b in wl.rounds[-1]]
     ms_l, _ = timed(lambda: run_all("loop"), loop_steps, 1)
     diff = max(float(torch.linalg.vector_norm(c.parent() - x) / torch.linalg.vector_norm(x).clamp_min(1e-300))
-               for (c, a, b), x in zip(round_tensors[-1], last_c))
+               for (c, a, b), x in zip(wl.rounds[-1], last_c))
+    del last_c
     out = {
-        "dtype": dtype, "pairs": n, "rounds": len(rounds), "output_budget_gib": budget_gb,
+        "dtype": dtype, "pairs": n, "rounds": len(wl.rounds), "output_budget_gib": budget_gb,
         "items": sum(i["items"] for i in infos), "shallow_items": sum(i["shallow_items"] for i in infos),
         "split_items": sum(i["split_items"] for i in infos), "workgroups": infos[0]["workgroups"],
         "graph": all(bt.info()["graph"] for bt in batches),
@@ -198,25 +208,41 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
         "autotune_s": round(tune_s, 2), "persistent_vs_loop_max_rel_diff": diff,
     }
     if cpu_sample:
-        # CPU baseline on a bounded sample: the oracle restatement on the first pairs
+        # CPU baseline on a bounded sample: the oracle restatement on the GPU's own
+        # inputs of seeded pairs, whose GPU outputs (copied after each round's
+        # persistent run) are checked against it
         from threadpoolctl import threadpool_limits
 
         from oracle import backend_base as bb
-        sample = pairs[:cpu_sample]
         rs = np.random.default_rng(seed + 2)
-        tc = 0.0
+        pick = set(int(i) for i in rs.choice(n, size=min(cpu_sample, n), replace=False))
+        where = {i: (r, j) for r, idx in enumerate(wl.round_pairs) for j, i in enumerate(idx)}
+        got = {}
+        for r, bt in enumerate(batches):
+            bt.run("persistent")
+            torch.cuda.synchronize()
+            for j, i in enumerate(wl.round_pairs[r]):
+                if i in pick:
+                    got[i] = wl.rounds[r][j][0].parent().cpu().numpy()
+        tc, worst = 0.0, 0.0
         with threadpool_limits(limits=threads):
-            for ia, ib, ic in sample:
-                A = (rs.standard_normal((2,) * len(ia)) + 1j * rs.standard_normal((2,) * len(ia)))
-                B = (rs.standard_normal((2,) * len(ib)) + 1j * rs.standard_normal((2,) * len(ib)))
+            for i in sorted(pick):
+                r, j = where[i]
+                c, a, b = wl.rounds[r][j]
+                ia, ib, ic = pairs[i]
+                A = a.parent().cpu().numpy().astype(np.complex128)
+                B = b.parent().cpu().numpy().astype(np.complex128)
                 t = time.perf_counter()
-                bb.binary_einsum_general(ic, A, ia, B, ib)
+                ref = bb.binary_einsum_general(ic, A, ia, B, ib)
                 tc += time.perf_counter() - t
-        out["cpu_sample_pairs"] = cpu_sample
-        out["cpu_pairs_per_s"] = round(cpu_sample / tc, 1)
+                worst = max(worst, rel(got[i], ref))
+        out["cpu_sample_pairs"] = len(pick)
+        out["cpu_pairs_per_s"] = round(len(pick) / tc, 1)
+        out["cpu_sample_threads"] = threads
+        out["sample_max_rel_err_vs_cpu_oracle"] = worst
     for bt in batches:
         bt.close()
-    del arena, pool
+    del wl
     torch.cuda.empty_cache()
     return out
 
@@ -520,7 +546,8 @@ def main() -> int:
     import muscle
     from muscle import workloads as wl
     dev = torch.device("cuda", 0)
-    threads = int(os.environ.get("MUSCLE_BENCH_CPU_THREADS", "16"))
+    facts = cpu_facts()
+    threads = int(os.environ.get("MUSCLE_BENCH_CPU_THREADS", facts["blas_threads"]))
     extra = {}
 
     # ---------------- headline: config 2, ComplexF64
@@ -536,11 +563,17 @@ def main() -> int:
     log(f"headline {ms:.3f} ms/step, {tflops:.2f} TFLOP/s")
     cpu, ref_c2 = None, None
     if not args.no_cpu_baseline:
-        log(f"cpu baseline: oracle BackendBase restatement, {threads} threads, median of {args.cpu_reps}")
+        log(f"cpu baseline: oracle BackendBase restatement, {threads} threads, median of {args.cpu_reps}; {facts}")
         t_cpu, ref_c2, blas = cpu_oracle(A, ia, B, ib, ic, threads, args.cpu_reps)
+        t_one, _, _ = cpu_oracle(A, ia, B, ib, ic, 1, 1, warm=False)
         cpu = {"value": round(flops / t_cpu / 1e12, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
-               "sample": f"the full config-2 ComplexF64 contraction (M=N=K=4096), median of {args.cpu_reps} runs "
-                         f"of {t_cpu:.3f} s; numpy {np.__version__}, BLAS {'; '.join(blas)}"}
+               "sample": f"the full config-2 ComplexF64 contraction (M=N=K=4096): BackendBase restatement "
+                         f"(permute, zgemm, permute), median of {args.cpu_reps} runs of {t_cpu:.3f} s after a "
+                         f"warm-up run at {threads} BLAS threads; 1 thread: {t_one:.2f} s (one run); "
+                         f"numpy {np.__version__}, BLAS {'; '.join(blas)}; {facts['cpu_model']}, "
+                         f"nproc {facts['nproc']}, {facts['physical_cores_available']} physical cores available",
+               "threads": threads, "value_1thread": round(flops / t_one / 1e12, 4), "seconds": round(t_cpu, 4),
+               "seconds_1thread": round(t_one, 3), "blas": "; ".join(blas), **facts}
         extra["headline_rel_err_vs_cpu_oracle"] = rel(c.parent().cpu().numpy(), ref_c2)
     plain_ms = ms
     del c
@@ -654,13 +687,14 @@ def main() -> int:
             res[dist] = row
         extra["ladder_c1"] = res
     if want("batched"):
-        try:
-            extra["batched_c64"] = bench_batched("c64", args.batch_pairs, 0, args.batch_budget_gib, 3, 1,
-                                                 0 if args.no_cpu_baseline else 300, threads)
-            log(f"batched: {extra['batched_c64']}")
-        except Exception as e:  # reported in the JSON, not hidden
-            extra["batched_c64"] = {"error": repr(e)}
-            log(f"batched failed: {e!r}")
+        for bdt in ("c64", "c32"):
+            try:
+                extra[f"batched_{bdt}"] = bench_batched(bdt, args.batch_pairs, 0, args.batch_budget_gib, 3, 1,
+                                                        0 if args.no_cpu_baseline else 300, threads)
+                log(f"batched {bdt}: {extra[f'batched_{bdt}']}")
+            except Exception as e:  # reported in the JSON, not hidden
+                extra[f"batched_{bdt}"] = {"error": repr(e)}
+                log(f"batched {bdt} failed: {e!r}")
     if want("served"):
         for dist, n_jobs in (("small", args.served_small_jobs), ("spec", args.served_jobs)):
             if n_jobs <= 0:

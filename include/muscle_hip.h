@@ -118,7 +118,11 @@ typedef struct muscle_hip_handle_s* muscle_hip_handle_t;
  * task-local handle (ext/MuscleCUDAExt.jl:36-38). */
 MUSCLE_HIP_API int muscle_hip_create(muscle_hip_handle_t* handle);
 MUSCLE_HIP_API int muscle_hip_destroy(muscle_hip_handle_t handle);
-/* Sets the hipStream_t all later calls on this handle enqueue on. */
+/* Sets the hipStream_t all later calls on this handle enqueue on.  When it
+ * differs from the current one, the new stream is first made to wait for
+ * everything the handle enqueued on the old stream (one event), because all
+ * calls of a handle share its workspace and plan tables; the old stream must
+ * still exist at this call.  Calls that must overlap use separate handles. */
 MUSCLE_HIP_API int muscle_hip_set_stream(muscle_hip_handle_t handle, void* stream);
 MUSCLE_HIP_API int muscle_hip_synchronize(muscle_hip_handle_t handle);
 MUSCLE_HIP_API const char* muscle_hip_status_string(int status);
@@ -266,6 +270,30 @@ MUSCLE_HIP_API int muscle_hip_batch_create(muscle_hip_handle_t handle, int64_t n
  * mode 2: one persistent launch without graph capture. */
 MUSCLE_HIP_API int muscle_hip_batch_run(muscle_hip_handle_t handle, muscle_hip_batch_t batch, int mode);
 MUSCLE_HIP_API int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t* info);
+
+/* How one pair of a batch runs: every launch of a batch run belongs to one
+ * of these routes. */
+typedef enum {
+  MUSCLE_HIP_PAIR_EMPTY = 0,   /* C has no elements: nothing runs */
+  MUSCLE_HIP_PAIR_TILES = 1,   /* output tiles of the persistent MFMA launch */
+  MUSCLE_HIP_PAIR_SPLIT_K = 2, /* few output tiles and a long K: K slices, summed in slice order */
+  MUSCLE_HIP_PAIR_SHALLOW = 3, /* K <= 16: the vector-FMA launch */
+  MUSCLE_HIP_PAIR_SOLO = 4     /* 2^29 or more index points: a single-contraction launch of its own */
+} muscle_hip_pair_route_t;
+
+typedef struct {
+  int32_t route;      /* muscle_hip_pair_route_t */
+  int32_t tile;       /* muscle_hip_tile_t of the TILES and SPLIT_K routes, -1 otherwise */
+  int32_t size_class; /* muscle_hip_tuning_class(m, n) */
+  int32_t ksplit;     /* K slices (1 unless SPLIT_K) */
+  int64_t m, n, k, batch; /* fused extents of the pair */
+} muscle_hip_batch_pair_info_t;
+
+/* The route and fused shape of pair `pair` (0-based, in the order given to
+ * muscle_hip_batch_create).  Replaces no reference function: it lets a caller
+ * (and the tests) see which launch computes each pair. */
+MUSCLE_HIP_API int muscle_hip_batch_get_pair_info(muscle_hip_batch_t batch, int64_t pair,
+                                                  muscle_hip_batch_pair_info_t* info);
 MUSCLE_HIP_API int muscle_hip_batch_destroy(muscle_hip_batch_t batch);
 
 /* Autotuner: for every size class present in the batch, times each tile shape

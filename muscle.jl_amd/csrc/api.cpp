@@ -83,6 +83,7 @@ Plan::~Plan() {
 }
 
 Handle::~Handle() {
+  if (order_event) (void)hipEventDestroy(order_event);
   if (ws) (void)hipFree(ws);
   cache.clear();  // plans free their device tables
   release_lt(*this);
@@ -707,9 +708,9 @@ int muscle_hip_create(muscle_hip_handle_t* handle) {
     if (!handle) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle pointer");
     int dev = 0;
     MH_HIP_CHECK(hipGetDevice(&dev));
-    Handle* h = new Handle();
+    std::unique_ptr<Handle> h(new Handle());
     h->device = dev;
-    *handle = reinterpret_cast<muscle_hip_handle_t>(h);
+    *handle = reinterpret_cast<muscle_hip_handle_t>(h.release());
   });
 }
 
@@ -727,7 +728,16 @@ int muscle_hip_set_stream(muscle_hip_handle_t handle, void* stream) {
     Handle* h = reinterpret_cast<Handle*>(handle);
     if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
     std::lock_guard<std::mutex> lk(h->mu);
-    h->stream = reinterpret_cast<hipStream_t>(stream);
+    const hipStream_t ns = reinterpret_cast<hipStream_t>(stream);
+    if (ns == h->stream) return;
+    // Every call of a handle shares its workspace (casts, packed operands,
+    // split-K partials, sliced planes) and its cached plans' tables, which
+    // the first call uploads asynchronously.  Work enqueued on the new stream
+    // therefore starts after all the handle has enqueued on the old one.
+    if (!h->order_event) MH_HIP_CHECK(hipEventCreateWithFlags(&h->order_event, hipEventDisableTiming));
+    MH_HIP_CHECK(hipEventRecord(h->order_event, h->stream));
+    MH_HIP_CHECK(hipStreamWaitEvent(ns, h->order_event, 0));
+    h->stream = ns;
   });
 }
 

@@ -78,6 +78,10 @@ constexpr int kBatchWgPerCu = 2;
 struct Batch {
   Handle* h = nullptr;
   int dtype = 0;
+  // complex product form (3: Gauss, 4: four products), read from the handle
+  // when the batch is built and used by every later run, so a graph captured
+  // once and its replays agree with the plain loop
+  int complex_mult = 3;
   std::vector<muscle_hip_tensor_t> cs, as, bs;  // for the per-pair plain loop
   std::vector<std::vector<int32_t>> modes;      // storage behind cs/as/bs
   std::vector<std::vector<int64_t>> ext, str;
@@ -97,6 +101,8 @@ struct Batch {
   bool allow_split = true;  // off for the autotuner's timing batches
   std::vector<int64_t> htab;
   std::vector<int> pair_class;
+  // per pair, in the order given: the route its tiles take and its fused shape
+  std::vector<muscle_hip_batch_pair_info_t> pinfo;
   void* dmem = nullptr;  // pairs | items | counter | tables
   BatchPair* dpairs = nullptr;
   int2* ditems = nullptr;

@@ -131,6 +131,20 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
   B.partial_bytes = 0;
   B.split_items = 0;
   const size_t es = dtype_size(B.dtype);
+  B.pinfo.assign((size_t)n, muscle_hip_batch_pair_info_t());
+  for (int64_t i = 0; i < n; ++i) {
+    const PlannedPair& L = *lw[(size_t)i];
+    muscle_hip_batch_pair_info_t& q = B.pinfo[(size_t)i];
+    std::memset(&q, 0, sizeof q);
+    q.route = L.d.empty ? MUSCLE_HIP_PAIR_EMPTY : L.solo ? MUSCLE_HIP_PAIR_SOLO : MUSCLE_HIP_PAIR_TILES;
+    q.tile = L.d.empty || L.solo ? -1 : L.tile;
+    q.size_class = tuning_class(L.d.M, L.d.N);
+    q.ksplit = 1;
+    q.m = L.d.M;
+    q.n = L.d.N;
+    q.k = L.d.K;
+    q.batch = L.d.B;
+  }
   std::vector<int2> shallow, sliced;
   for (int64_t i : order) {
     const PlannedPair& L = *lw[(size_t)i];
@@ -162,6 +176,13 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     }
     B.ksplit[(size_t)i] = (int32_t)ks;
     B.kchunk[(size_t)i] = (int32_t)kc;
+    B.pinfo[(size_t)i].ksplit = (int32_t)ks;
+    if (sh) {
+      B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SHALLOW;
+      B.pinfo[(size_t)i].tile = -1;
+    } else if (ks > 1) {
+      B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SPLIT_K;
+    }
     std::vector<int2>& dst = sh ? shallow : ks > 1 ? sliced : B.hitems;
     for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
     if (!sh) used[L.tile] = true;
@@ -294,6 +315,7 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
                                    const muscle_hip_tensor_t* bs, const int32_t* tiles, bool allow_split) {
   std::unique_ptr<Batch> B(new Batch());
   B->allow_split = allow_split;
+  B->complex_mult = h.complex_mult;
   std::vector<PlannedPair> lw;
   plan_batch(h, n, cs, as, bs, tiles, true, *B, lw);
   const BatchLayout l = batch_layout(*B, n);
@@ -336,7 +358,7 @@ void batch_launch(Batch& b, hipStream_t s) {
     a.tilecnt = b.dtilecnt;
     a.ntilecnt = b.ntilecnt;
     a.nitems = nmain;
-    a.complex3m = b.h->complex_mult == 3;
+    a.complex3m = b.complex_mult == 3;
     size_t lds = b.lds;
     a.item_off = (int32_t)batch_item_offset(is_double(b.dtype), is_complex(b.dtype), lds);
     BatchArgs sp = a;
@@ -353,8 +375,20 @@ void batch_launch(Batch& b, hipStream_t s) {
   }
 }
 
+namespace {
+// Sets the handle's complex product form for the lifetime of the object
+// (the batch's own form for its plain-loop and solo launches).
+struct ProductForm {
+  Handle& h;
+  int saved;
+  ProductForm(Handle& h_, int form) : h(h_), saved(h_.complex_mult) { h.complex_mult = form; }
+  ~ProductForm() { h.complex_mult = saved; }
+};
+}  // namespace
+
 void batch_run(Batch& b, int mode) {
   Handle& h = *b.h;
+  ProductForm form(h, b.complex_mult);
   if (mode == 1) {
     for (size_t i = 0; i < b.cs.size(); ++i) {
       Plan* p = get_plan(h, 1, MUSCLE_HIP_PRECISION_NATIVE, 1e-12, &b.cs[i], &b.as[i], &b.bs[i]);
@@ -412,6 +446,7 @@ void batch_run(Batch& b, int mode) {
 // Times every tile shape on up to `samples` pairs of each size class.
 void batch_autotune(Batch& b, int samples, int reps) {
   Handle& h = *b.h;
+  ProductForm form(h, b.complex_mult);
   std::vector<std::vector<int64_t>> by_class(Handle::kTuneClasses);
   for (size_t i = 0; i < b.hpairs.size(); ++i) {
     const BatchPair& P = b.hpairs[i];
@@ -519,6 +554,15 @@ int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t*
     info->bytes = b->bytes;
     info->table_bytes = (int64_t)(b->htab.size() * 8 + b->hpairs.size() * sizeof(BatchPair) +
                                   b->hitems.size() * sizeof(int2));
+  });
+}
+
+int muscle_hip_batch_get_pair_info(muscle_hip_batch_t batch, int64_t pair, muscle_hip_batch_pair_info_t* info) {
+  return guarded_b([&] {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b || !info) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null batch or info");
+    if (pair < 0 || pair >= (int64_t)b->pinfo.size()) fail(MUSCLE_HIP_INVALID_ARGUMENT, "pair index out of range");
+    *info = b->pinfo[(size_t)pair];
   });
 }
 

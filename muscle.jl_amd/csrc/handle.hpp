@@ -86,6 +86,10 @@ struct Handle {
   int device = 0;
   int cus = 0;  // compute units of the device (queried once)
   hipStream_t stream = nullptr;
+  // recorded on the old stream when the stream changes; the new stream waits
+  // for it, so calls on different streams never share the workspace, the
+  // split-K partials or a plan's tables at the same time
+  hipEvent_t order_event = nullptr;
   std::mutex mu;
   std::unordered_map<std::string, std::unique_ptr<Plan>> cache;
   std::list<std::string> lru;

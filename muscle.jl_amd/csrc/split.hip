@@ -521,6 +521,22 @@ int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // log-uniform (1e-4..1e4) and unit-norm-row inputs (not a rigorous bound).
 double ozaki_model(int S) { return 0.4 * std::pow(255.0, -(double)(S - 1)); }
 
+// A-priori norm-wise relative error estimates of the rungs that accumulate in
+// FP32 (u = 2^-24, FP32's unit roundoff; the rounding errors of an n-term FP32
+// sum grow like sqrt(n) u on data of random signs):
+//   FP32    operands rounded to FP32 (2u), the whole K summed in FP32;
+//   BF16X3  three bf16 terms (24 significand bits) and the dropped cross
+//           products (about 4u), FP32 sums over 1024-long K chunks;
+//   FP16X2  two fp16 terms (22 bits: 2^-22 per operand), FP32 sums over
+//           1024-long K chunks.
+double fp32_model(int64_t K) { return std::ldexp(1.0, -24) * (2.0 + std::sqrt((double)K)); }
+double bf16x3_model(int64_t K) {
+  return std::ldexp(1.0, -24) * (4.0 + std::sqrt((double)std::min<int64_t>(K, 1024)));
+}
+double fp16x2_model(int64_t K) {
+  return std::ldexp(1.0, -21) + std::ldexp(1.0, -24) * std::sqrt((double)std::min<int64_t>(K, 1024));
+}
+
 struct Product {
   int a_plane, b_plane;
   double sre, sim;  // contribution signs to the real / imaginary part
@@ -570,7 +586,7 @@ void prepare_split(Plan& p, double target) {
   if (p.precision == MUSCLE_HIP_PRECISION_FP32) {
     p.kind = Plan::GEMM;
     p.fp32_math = true;
-    p.err_bound = 1e-6;
+    p.err_bound = fp32_model(d.K);
     plan_gemm_ksplit(p);
     return;
   }
@@ -596,14 +612,14 @@ void prepare_split(Plan& p, double target) {
       p.split_mode = MODE_BF16;
       p.esize = 2;
       p.slices = 3;
-      p.err_bound = 1e-6;
+      p.err_bound = bf16x3_model(d.K);
       kchunk = 1024;  // FP32 accumulation per 1024-long K chunk, chunks added in FP64
       break;
     case MUSCLE_HIP_PRECISION_FP16X2:
       p.split_mode = MODE_F16;
       p.esize = 2;
       p.slices = 2;
-      p.err_bound = 1e-5;
+      p.err_bound = fp16x2_model(d.K);
       kchunk = 1024;
       break;
     default: fail(MUSCLE_HIP_INVALID_ARGUMENT, "unknown precision mode");

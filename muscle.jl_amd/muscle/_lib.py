@@ -59,6 +59,26 @@ class BatchInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchPairInfo(ctypes.Structure):
+    _fields_ = [
+        ("route", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("size_class", ctypes.c_int32),
+        ("ksplit", ctypes.c_int32),
+        ("m", ctypes.c_int64),
+        ("n", ctypes.c_int64),
+        ("k", ctypes.c_int64),
+        ("batch", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# batched pair routes (muscle_hip_pair_route_t)
+PAIR_ROUTES = {0: "empty", 1: "tiles", 2: "split_k", 3: "shallow", 4: "solo"}
+
+
 class JobDesc(ctypes.Structure):
     _fields_ = [
         ("n_leaves", ctypes.c_int32),
@@ -143,6 +163,8 @@ def _load() -> ctypes.CDLL:
                                      ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
         "muscle_hip_batch_run": ([H, ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
         "muscle_hip_batch_get_info": ([ctypes.c_void_p, ctypes.POINTER(BatchInfo)], ctypes.c_int),
+        "muscle_hip_batch_get_pair_info": ([ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(BatchPairInfo)],
+                                           ctypes.c_int),
         "muscle_hip_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "muscle_hip_batch_autotune": ([H, ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "muscle_hip_tuning_save": ([H, ctypes.c_char_p], ctypes.c_int),
@@ -160,7 +182,7 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_server_clock": ([ctypes.c_void_p], ctypes.c_double),
         "muscle_hip_server_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "muscle_hip_svd_thin": ([ctypes.c_void_p] + [ctypes.POINTER(TensorDesc)] * 4, ctypes.c_int),
-    "muscle_hip_qr_thin": ([ctypes.c_void_p] + [ctypes.POINTER(TensorDesc)] * 3, ctypes.c_int),
+        "muscle_hip_qr_thin": ([ctypes.c_void_p] + [ctypes.POINTER(TensorDesc)] * 3, ctypes.c_int),
         "muscle_hip_describe": ([T, T, T, ctypes.c_int, ctypes.c_double, I, ctypes.POINTER(ctypes.c_int32),
                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     }

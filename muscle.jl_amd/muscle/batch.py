@@ -50,9 +50,8 @@ class ContractionBatch:
         _lib.check(_lib.lib.muscle_hip_batch_create(self._h, n, cs, as_, bs, tl, ctypes.byref(self._b)))
 
     def run(self, mode: str = "persistent") -> None:
+        # the library runs the batch in the complex product form it was built with
         h = _handle_for(self.device)
-        # the batch's complex product form, whatever calls ran on the handle since
-        _lib.set_option(self.device, h, 0, self._complex_mult)
         _lib.check(_lib.lib.muscle_hip_batch_run(h, self._b, MODES[mode]))
 
     def info(self) -> dict:
@@ -60,9 +59,17 @@ class ContractionBatch:
         _lib.check(_lib.lib.muscle_hip_batch_get_info(self._b, ctypes.byref(inf)))
         return inf.as_dict()
 
+    def pair_info(self, i: int) -> dict:
+        """Route (empty, tiles, split_k, shallow or solo), tile, size class, K slices and
+        fused shape of pair i."""
+        inf = _lib.BatchPairInfo()
+        _lib.check(_lib.lib.muscle_hip_batch_get_pair_info(self._b, int(i), ctypes.byref(inf)))
+        out = inf.as_dict()
+        out["route"] = _lib.PAIR_ROUTES[out["route"]]
+        return out
+
     def autotune(self, samples: int = 16, reps: int = 3) -> None:
         h = _handle_for(self.device)
-        _lib.set_option(self.device, h, 0, self._complex_mult)
         _lib.check(_lib.lib.muscle_hip_batch_autotune(h, self._b, int(samples), int(reps)))
 
     def close(self) -> None:

@@ -133,6 +133,67 @@ def circuit_pairs(n: int = 10000, seed: int = 0, ranks=(4, 22), contracted=(1, 1
     return out
 
 
+class CircuitBatchWorkload:
+    """The batched uneven-contraction workload on the device: `pairs` (index
+    structure from circuit_pairs) split into `rounds`, each a list of
+    (c, a, b) muscle.Tensors whose outputs together stay within the output
+    budget.  Inputs are column-major dims-2 views at seeded offsets into one
+    random pool (read-only, as a tree reuses its tensors); every round's
+    outputs are views into one arena, so a round overwrites the previous
+    round's outputs."""
+
+    def __init__(self, pairs, rounds, round_pairs, pool, arena):
+        self.pairs, self.rounds, self.round_pairs, self.pool, self.arena = pairs, rounds, round_pairs, pool, arena
+
+
+def circuit_batch_workload(dtype: str = "c64", n: int = 10000, seed: int = 0, budget_gib: float = 96.0,
+                           device=None) -> CircuitBatchWorkload:
+    """n pairwise contractions of a circuit-like network (every index of dimension 2,
+    ranks 4-22, 1-10 contracted and 0-2 hyper indices, outputs of 2^4..2^24
+    elements), ComplexF64 ("c64") or ComplexF32 ("c32"), entries iid complex normal
+    from a seeded device generator.  round_pairs[r] lists the pair indices of round r."""
+    import torch
+
+    from .tensor import Tensor
+    dev = device if device is not None else torch.device("cuda", 0)
+    tdt = torch.complex128 if dtype == "c64" else torch.complex64
+    esz = 16 if dtype == "c64" else 8
+    pairs = circuit_pairs(n, seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pool_n = 1 << 23
+    pool = torch.randn(pool_n, dtype=tdt, device=dev, generator=g)
+    rng = np.random.default_rng(seed + 1)
+    budget = int(budget_gib * 2 ** 30) // esz
+    round_idx, cur, used = [], [], 0
+    for i, p in enumerate(pairs):
+        sz = 1 << len(p[2])
+        if cur and used + sz > budget:
+            round_idx.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += sz
+    round_idx.append(cur)
+    arena_n = max(sum(1 << len(pairs[i][2]) for i in r) for r in round_idx)
+    arena = torch.empty(arena_n, dtype=tdt, device=dev)
+
+    def dims2(base, rank, off):
+        return base.as_strided((2,) * rank, tuple(1 << i for i in range(rank)), off)
+
+    rounds = []
+    for r in round_idx:
+        ts, off = [], 0
+        for i in r:
+            ia, ib, ic = pairs[i]
+            a = Tensor(dims2(pool, len(ia), int(rng.integers(0, pool_n - (1 << len(ia)) + 1))), ia)
+            b = Tensor(dims2(pool, len(ib), int(rng.integers(0, pool_n - (1 << len(ib)) + 1))), ib)
+            c = Tensor(dims2(arena, len(ic), off), ic)
+            off += 1 << len(ic)
+            ts.append((c, a, b))
+        rounds.append(ts)
+    return CircuitBatchWorkload(pairs, rounds, round_idx, pool, arena)
+
+
 def pair_flops(pair, complex_=True) -> float:
     ia, ib, ic = pair
     labels = set(ia) | set(ib)

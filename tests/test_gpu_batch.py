@@ -101,6 +101,53 @@ def test_complex_products(mult):
     check(pairs, refs, 1e-12)
 
 
+def test_batch_keeps_its_complex_product_form():
+    """A batch runs in the product form the handle had when it was built, also
+    after the handle's option changes (graph replays and the plain loop agree
+    bit for bit with a batch built and run under that form alone)."""
+    from muscle import _lib
+    from muscle.ops import _handle_for
+    rng = np.random.default_rng(17)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=40)
+    dev = pairs[0][1].device.index or 0
+    out = {}
+    for mult in (4, 3):
+        batch = ContractionBatch(pairs, complex_mult=mult)
+        _lib.set_option(dev, _handle_for(dev), 0, 7 - mult)  # the other form, set after the build
+        for mode in ("persistent", "persistent", "loop"):
+            for c, a, b in pairs:
+                c.parent().fill_(float("nan"))
+            batch.run(mode)
+            torch.cuda.synchronize()
+            got = [c.parent().clone() for c, a, b in pairs]
+            key = (mult, mode)
+            if key in out:
+                assert all(torch.equal(x, y) for x, y in zip(out[key], got)), key
+            out[key] = got
+        check(pairs, refs, 1e-12)
+        batch.close()
+    # 3M and 4M give different roundings: the form was applied, not ignored
+    assert any(not torch.equal(x, y) for x, y in zip(out[(3, "persistent")], out[(4, "persistent")]))
+
+
+def test_pair_routes_cover_the_launches():
+    rng = np.random.default_rng(18)
+    pairs, refs = make_pairs(rng, np.complex128, n_circuit=60)
+    batch = ContractionBatch(pairs)
+    info = batch.info()
+    routes = [batch.pair_info(i) for i in range(len(pairs))]
+    assert sum(r["route"] == "shallow" for r in routes) > 0
+    for r in routes:
+        assert r["route"] in ("empty", "tiles", "split_k", "shallow", "solo")
+        assert (r["tile"] >= 0) == (r["route"] in ("tiles", "split_k"))
+        assert (r["ksplit"] > 1) == (r["route"] == "split_k")
+        assert r["size_class"] == tuning_class(r["m"], r["n"])
+    with pytest.raises(muscle.ArgumentError):
+        batch.pair_info(len(pairs))
+    assert info["pairs"] == len(pairs)
+    batch.close()
+
+
 def test_autotune_and_tuning_file(tmp_path):
     rng = np.random.default_rng(15)
     pairs, refs = make_pairs(rng, np.complex128, n_circuit=80)

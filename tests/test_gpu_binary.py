@@ -286,6 +286,32 @@ def test_handle_follows_device_forms_and_streams():
         assert rel_err(got, ref) <= 1e-13
 
 
+def test_stream_switch_orders_shared_workspace():
+    """Calls of one handle on two streams, with no synchronization between them:
+    both run split-K (their partial sums share the handle's workspace), so the
+    call on the second stream must start only after the first stream's call.
+    Repeated with alternating streams; every result against the oracle."""
+    rng = np.random.default_rng(33)
+    K = 1 << 17
+    ops = []
+    for _ in range(2):
+        A, B = randn(rng, (128, K)), randn(rng, (K, 96))
+        ops.append((dev_tensor(A, ["i", "j"]), dev_tensor(B, ["j", "k"]), A @ B))
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    outs = []
+    for rep in range(6):
+        stream = side if rep % 2 == 0 else torch.cuda.default_stream()
+        a, b, ref = ops[rep % 2]
+        with torch.cuda.stream(stream):
+            c = muscle.binary_einsum(a, b)
+            c.parent().record_stream(stream)
+        outs.append((c, ref))
+    torch.cuda.synchronize()
+    for c, ref in outs:
+        assert rel_err(c.parent().cpu().numpy(), ref) <= 1e-12
+
+
 @pytest.mark.parametrize("T", [np.complex128, np.float32])
 def test_contracted_extent_zero(T):
     """A contracted mode of extent 0 sums nothing: C is all zeros (C's memory
