@@ -166,6 +166,29 @@ MUSCLE_HIP_API int muscle_hip_binary_einsum(muscle_hip_handle_t handle,
                                             int precision, double target_rel_error,
                                             muscle_hip_info_t* info);
 
+/* The same contraction as muscle_hip_binary_einsum with NATIVE arithmetic,
+ * enqueued without allocating any device memory: no hipMalloc, hipFree or
+ * allocating library call on any call, the first included.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The contraction's offset tables and its workspace (split-K
+ * or reduction partial sums, casts of mixed-precision operands) live in the
+ * library's static device scratch of 32 MiB (33,554,432 bytes) per GPU, held
+ * from library load until the process ends; operands are gathered where they
+ * lie (no packed copies).  A contraction that needs more scratch is refused
+ * with MUSCLE_HIP_INVALID_ARGUMENT and a message giving both sizes.  Every
+ * call on one GPU shares that scratch and leaves its contents behind: calls
+ * on one handle are ordered by its stream (muscle_hip_set_stream orders a new
+ * stream after the old one), but calls from different handles on one GPU
+ * that overlap in time race on it.  The offset tables are copied from host
+ * memory on each call.  Results are those of muscle_hip_binary_einsum on the
+ * same plan. */
+MUSCLE_HIP_API int muscle_hip_binary_einsum_noalloc(muscle_hip_handle_t handle,
+                                                    const muscle_hip_tensor_t* c,
+                                                    const muscle_hip_tensor_t* a,
+                                                    const muscle_hip_tensor_t* b,
+                                                    muscle_hip_info_t* info);
+
 /* C = A .* broadcast(B), with modes(B) a subset of modes(A) = modes(C), in
  * any relative order; C has A's element type (copy(a), hadamard.jl:38) and
  * may alias A (hadamard!(a, a, b), test/unit/operations/hadamard.jl:14-17).

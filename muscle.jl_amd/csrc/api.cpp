@@ -710,6 +710,7 @@ int muscle_hip_create(muscle_hip_handle_t* handle) {
     MH_HIP_CHECK(hipGetDevice(&dev));
     std::unique_ptr<Handle> h(new Handle());
     h->device = dev;
+    h->scratch = scratch_address(dev);
     *handle = reinterpret_cast<muscle_hip_handle_t>(h.release());
   });
 }
@@ -797,6 +798,45 @@ int muscle_hip_binary_einsum(muscle_hip_handle_t handle, const muscle_hip_tensor
     std::lock_guard<std::mutex> lk(h->mu);
     Plan* p = get_plan(*h, 1, precision, target_rel_error, c, a, b);
     run_contraction(*h, *p, *c, *a, b);
+    fill_info(info, *p, c, a, b);
+  });
+}
+
+int muscle_hip_binary_einsum_noalloc(muscle_hip_handle_t handle, const muscle_hip_tensor_t* c,
+                                     const muscle_hip_tensor_t* a, const muscle_hip_tensor_t* b,
+                                     muscle_hip_info_t* info) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle");
+    check_tensor(c, "C");
+    check_tensor(a, "A");
+    check_tensor(b, "B");
+    std::lock_guard<std::mutex> lk(h->mu);
+    // host planning only: the plan's tables go to the scratch, not to a
+    // cached device allocation, and operands are gathered where they lie
+    // (no packed copies)
+    std::unique_ptr<Plan> p = build_contraction_plan(lower_binary(*c, *a, *b), MUSCLE_HIP_PRECISION_NATIVE);
+    if (p->kind != Plan::EMPTY && p->kind != Plan::GEMM && p->kind != Plan::REDUCE)
+      fail(MUSCLE_HIP_UNSUPPORTED, "binary_einsum_noalloc: unsupported contraction plan");
+    const size_t tab = p->kind == Plan::EMPTY ? 0 : align_up(p->htab.size() * sizeof(int64_t));
+    const size_t need = tab + contraction_workspace_bytes(*p, *a, b);
+    if (need > kScratchBytes)
+      fail(MUSCLE_HIP_INVALID_ARGUMENT, "binary_einsum_noalloc: this contraction needs " + std::to_string(need) +
+                                            " bytes of scratch (offset tables, partial sums, casts); the maximum is " +
+                                            std::to_string(kScratchBytes));
+    if (p->kind != Plan::EMPTY) {
+      // stream-ordered after the previous call's kernels, which may still read the old tables
+      MH_HIP_CHECK(hipMemcpyAsync(h->scratch, p->htab.data(), p->htab.size() * sizeof(int64_t),
+                                  hipMemcpyHostToDevice, h->stream));
+      p->dtab = reinterpret_cast<int64_t*>(h->scratch);
+    }
+    try {
+      run_contraction(*h, *p, *c, *a, b, h->scratch + tab, kScratchBytes - tab);
+    } catch (...) {
+      p->dtab = nullptr;  // borrowed: the plan must not free the scratch
+      throw;
+    }
+    p->dtab = nullptr;
     fill_info(info, *p, c, a, b);
   });
 }

@@ -81,9 +81,16 @@ struct Plan {
   ~Plan();
 };
 
+// Size of the static device scratch of muscle_hip_binary_einsum_noalloc
+// (scratch.hip), and its device address on `device` (resolved once per device,
+// with that device current).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+char* scratch_address(int device);
+
 struct Handle {
   static constexpr size_t kMaxPlans = 8192;
   int device = 0;
+  char* scratch = nullptr;  // the static scratch's address on this device
   int cus = 0;  // compute units of the device (queried once)
   hipStream_t stream = nullptr;
   // recorded on the old stream when the stream changes; the new stream waits

@@ -157,6 +157,7 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_plan_cache_size": ([H], ctypes.c_int64),
         "muscle_hip_plan_cache_clear": ([H], ctypes.c_int),
         "muscle_hip_binary_einsum": ([H, T, T, T, ctypes.c_int, ctypes.c_double, I], ctypes.c_int),
+        "muscle_hip_binary_einsum_noalloc": ([H, T, T, T, I], ctypes.c_int),
         "muscle_hip_hadamard": ([H, T, T, T, I], ctypes.c_int),
         "muscle_hip_unary_einsum": ([H, T, T, I], ctypes.c_int),
         "muscle_hip_batch_create": ([H, ctypes.c_int64, T, T, T, ctypes.POINTER(ctypes.c_int32),
