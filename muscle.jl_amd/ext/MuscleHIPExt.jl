@@ -15,17 +15,30 @@ backend methods the front ends call:
   unary_einsum(::BackendHIP, inds_y, x)       (src/Operations/unary_einsum.jl:35)
   unary_einsum!(::BackendHIP, y, x)           (:44)
 
-and, for the two-site gate update that follows the contractions, the
-factorization behind it:
+and the factorizations of the two-site gate update (whose generic method,
+src/Operations/simple_update.jl:35-82, runs on device tensors once the rule
+below selects this backend):
 
-  tensor_svd_thin(::BackendHIP, A; inds_u, inds_v, ind_s)   (src/Operations/tensor_svd.jl:59-68)
-  tensor_svd_trunc(::BackendHIP, A; ..., threshold, maxdim)  (:160-215)
-  tensor_qr_thin(::BackendHIP, A; inds_q, inds_r, ind_virtual) (src/Operations/tensor_qr.jl:27-68)
-  simple_update(::BackendHIP, A, ...)                        (src/Operations/simple_update.jl:37-87)
+  tensor_svd_thin(::BackendHIP, A; inds_u, inds_v, ind_s)      (src/Operations/tensor_svd.jl:59-68,100-125)
+  tensor_svd_thin!(::BackendHIP, U, s, V, A)                    (:72-79,129)
+  tensor_svd_trunc(::BackendHIP, A; ..., threshold, maxdim)     (:81-95,160-215)
+  tensor_svd_trunc!(::BackendHIP, U, s, V, A; ...)              (:217)
+  tensor_qr_thin(::BackendHIP, A; inds_q, inds_r, ind_virtual)  (src/Operations/tensor_qr.jl:27-68)
+  tensor_qr_thin!(::BackendHIP, Q, R, A)                        (:36-43,70)
 
-It assumes `struct BackendHIP <: Backend end` next to the other backends
-(src/Backend.jl:6-14) and `struct DomainROCm <: Domain end` next to the other
-domains (src/Domain.jl:6-9).  Set ENV["MUSCLE_HIP_LIB"] to the library path.
+`BackendHIP` carries the arithmetic of binary_einsum, which the front end
+cannot pass as keyword arguments (the CUDA extension forwards its kwargs to
+cuTENSOR instead, ext/MuscleCUDAExt.jl:22,37):
+
+  with_backend(BackendHIP(precision=:ozaki_int8, target_rel_error=1e-10)) do
+      binary_einsum(a, b)
+  end
+
+precision is one of :native (default), :fp32, :bf16x3, :fp16x2, :ozaki_int8;
+complex_mult is 3 (Gauss, default) or 4.  The backend methods also take them
+as keyword arguments.  The extension assumes the definitions shown in
+INTEGRATION.md next to the other backends (src/Backend.jl:6-14) and domains
+(src/Domain.jl:6-9).  Set ENV["MUSCLE_HIP_LIB"] to the library path.
 """
 module MuscleHIPExt
 
@@ -74,20 +87,41 @@ function check(status::Integer)
     error("libmuscle_hip: $name: $msg")
 end
 
-# One library handle per task and device; the stream is AMDGPU's task-local
-# stream, set before every call (binary_einsum's backend override is
-# task-scoped too, src/Backend.jl:16-18).
-function handle()
-    key = (:muscle_hip_handle, AMDGPU.device_id(AMDGPU.device()))
-    h = get(task_local_storage(), key, C_NULL)
-    if h == C_NULL
+# Library handles, pooled per device.  A call checks a handle out for its
+# duration, so no two tasks use one handle at once (a handle's stream is set
+# before every call and its workspace is shared by its calls), and checks it
+# back in; handles outlive the tasks that used them and are destroyed at exit.
+# The stream is AMDGPU's task-local stream; the library orders a handle's new
+# stream after the one it last used.
+const HANDLES = Dict{Int,Vector{Ptr{Cvoid}}}()
+const HANDLES_LOCK = ReentrantLock()
+
+function with_handle(f)
+    dev = AMDGPU.device_id(AMDGPU.device())
+    h = lock(HANDLES_LOCK) do
+        free = get!(HANDLES, dev, Ptr{Cvoid}[])
+        isempty(free) || return pop!(free)
         ref = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:muscle_hip_create, LIB), Cint, (Ref{Ptr{Cvoid}},), ref))
-        h = ref[]
-        task_local_storage(key, h)
+        return ref[]
     end
-    check(ccall((:muscle_hip_set_stream, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), h, AMDGPU.stream().stream))
-    return h
+    try
+        check(ccall((:muscle_hip_set_stream, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), h, AMDGPU.stream().stream))
+        return f(h)
+    finally
+        lock(() -> push!(HANDLES[dev], h), HANDLES_LOCK)
+    end
+end
+
+function __init__()
+    atexit() do
+        lock(HANDLES_LOCK) do
+            for hs in values(HANDLES), h in hs
+                ccall((:muscle_hip_destroy, LIB), Cint, (Ptr{Cvoid},), h)
+            end
+            empty!(HANDLES)
+        end
+    end
 end
 
 # Mode labels: Dict(ind => i for (i, ind) in enumerate(unique(...))), as
@@ -106,15 +140,24 @@ function describe(t::Tensor, labels::Dict{Index,Int32})
     return desc, (modes, exts, strs)
 end
 
-function contract!(c::Tensor, a::Tensor, b::Tensor; precision::Integer=0, target::Real=1e-12)
+const PRECISION = Dict{Symbol,Cint}(:native => 0, :fp32 => 1, :bf16x3 => 2, :fp16x2 => 3, :ozaki_int8 => 4)
+
+function contract!(c::Tensor, a::Tensor, b::Tensor; precision::Symbol=:native, target_rel_error::Real=1e-12,
+                   complex_mult::Integer=3)
+    prec = get(PRECISION, precision, nothing)
+    isnothing(prec) && throw(ArgumentError("unknown precision $precision"))
+    complex_mult in (3, 4) || throw(ArgumentError("complex_mult must be 3 or 4"))
     labels = modemap(a, b, c)
     dc, kc = describe(c, labels)
     da, ka = describe(a, labels)
     db, kb = describe(b, labels)
-    GC.@preserve c a b kc ka kb begin
-        check(ccall((:muscle_hip_binary_einsum, LIB), Cint,
-                    (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Cint, Cdouble, Ptr{CInfo}),
-                    handle(), dc, da, db, precision, target, C_NULL))
+    with_handle() do h
+        check(ccall((:muscle_hip_set_option, LIB), Cint, (Ptr{Cvoid}, Cint, Int64), h, 0, complex_mult))
+        GC.@preserve c a b kc ka kb begin
+            check(ccall((:muscle_hip_binary_einsum, LIB), Cint,
+                        (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Cint, Cdouble, Ptr{CInfo}),
+                        h, dc, da, db, prec, target_rel_error, C_NULL))
+        end
     end
     return c
 end
@@ -131,27 +174,31 @@ Muscle.choose_backend_rule(::typeof(hadamard!), ::DomainROCm, ::DomainROCm, ::Do
 Muscle.choose_backend_rule(::typeof(unary_einsum), ::DomainROCm) = BackendHIP()
 Muscle.choose_backend_rule(::typeof(unary_einsum!), ::DomainROCm, ::DomainROCm) = BackendHIP()
 
-function Muscle.binary_einsum(::BackendHIP, inds_c, a::Tensor, b::Tensor)
+function Muscle.binary_einsum(be::BackendHIP, inds_c, a::Tensor, b::Tensor; precision=be.precision,
+                              target_rel_error=be.target_rel_error, complex_mult=be.complex_mult)
     sizes = Dict{Index,Int}()
     for t in (a, b), (i, n) in zip(inds(t), size(parent(t)))
         sizes[i] = n
     end
     T = Base.promote_eltype(parent(a), parent(b))
     c = Tensor(similar(parent(a), T, Tuple(sizes[i] for i in inds_c)), collect(inds_c))
-    return contract!(c, a, b)
+    return contract!(c, a, b; precision, target_rel_error, complex_mult)
 end
 
-Muscle.binary_einsum!(::BackendHIP, c::Tensor, a::Tensor, b::Tensor) = contract!(c, a, b)
+Muscle.binary_einsum!(be::BackendHIP, c::Tensor, a::Tensor, b::Tensor; precision=be.precision,
+                      target_rel_error=be.target_rel_error, complex_mult=be.complex_mult) =
+    contract!(c, a, b; precision, target_rel_error, complex_mult)
 
 function Muscle.hadamard!(::BackendHIP, c::Tensor, a::Tensor, b::Tensor)
     labels = modemap(a, b)
     dc, kc = describe(c, labels)
     da, ka = describe(a, labels)
     db, kb = describe(b, labels)
-    GC.@preserve c a b kc ka kb begin
-        check(ccall((:muscle_hip_hadamard, LIB), Cint,
-                    (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Ptr{CInfo}),
-                    handle(), dc, da, db, C_NULL))
+    with_handle() do h
+        GC.@preserve c a b kc ka kb begin
+            check(ccall((:muscle_hip_hadamard, LIB), Cint,
+                        (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Ptr{CInfo}), h, dc, da, db, C_NULL))
+        end
     end
     return c
 end
@@ -164,9 +211,11 @@ function Muscle.unary_einsum!(::BackendHIP, y::Tensor, x::Tensor)
     labels = modemap(x)
     dy, ky = describe(y, labels)
     dx, kx = describe(x, labels)
-    GC.@preserve y x ky kx begin
-        check(ccall((:muscle_hip_unary_einsum, LIB), Cint, (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ptr{CInfo}),
-                    handle(), dy, dx, C_NULL))
+    with_handle() do h
+        GC.@preserve y x ky kx begin
+            check(ccall((:muscle_hip_unary_einsum, LIB), Cint, (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ptr{CInfo}),
+                        h, dy, dx, C_NULL))
+        end
     end
     return y
 end
@@ -180,7 +229,30 @@ end
 # --- factorization ---------------------------------------------------------------
 
 Muscle.choose_backend_rule(::typeof(Muscle.tensor_svd_thin), ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_svd_thin!), ::Vararg{DomainROCm,4}) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_svd_trunc), ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_svd_trunc!), ::Vararg{DomainROCm,4}) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_qr_thin), ::DomainROCm) = BackendHIP()
+Muscle.choose_backend_rule(::typeof(Muscle.tensor_qr_thin!), ::DomainROCm, ::DomainROCm, ::DomainROCm) = BackendHIP()
+# simple_update's generic method (src/Operations/simple_update.jl:35-82) then
+# runs its contractions, SVD and absorption through this backend
 Muscle.choose_backend_rule(::typeof(Muscle.simple_update), ::DomainROCm, ::DomainROCm, ::DomainROCm) = BackendHIP()
+
+# muscle_hip_svd_thin into caller-owned U[inds_u..., ind_s], s[ind_s], Vt[inds_v..., ind_s]
+function svd_into!(U::Tensor, s::Tensor, Vt::Tensor, A::Tensor)
+    labels = modemap(U, s, Vt, A)
+    du, ku = describe(U, labels)
+    ds, ks = describe(s, labels)
+    dv, kv = describe(Vt, labels)
+    da, ka = describe(A, labels)
+    with_handle() do h
+        GC.@preserve U s Vt A ku ks kv ka begin
+            check(ccall((:muscle_hip_svd_thin, LIB), Cint,
+                        (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}), h, du, ds, dv, da))
+        end
+    end
+    return U, s, Vt
+end
 
 # U[inds_u..., ind_s], s[ind_s], Vt[inds_v..., ind_s] with A = U s Vt, the
 # layout of BackendBase's tensor_svd_thin (src/Operations/tensor_svd.jl:100-125)
@@ -196,20 +268,69 @@ function Muscle.tensor_svd_thin(::BackendHIP, A::Tensor; inds_u=(), inds_v=(), i
     U = Tensor(similar(parent(A), T, su..., k), [inds_u; ind_s])
     s = Tensor(similar(parent(A), real(T), k), [ind_s])
     Vt = Tensor(similar(parent(A), T, sv..., k), [inds_v; ind_s])
-    labels = modemap(U, s, Vt, A)
-    du, ku = describe(U, labels)
-    ds, ks = describe(s, labels)
-    dv, kv = describe(Vt, labels)
-    da, ka = describe(A, labels)
-    GC.@preserve U s Vt A ku ks kv ka begin
-        check(ccall((:muscle_hip_svd_thin, LIB), Cint,
-                    (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}),
-                    handle(), du, ds, dv, da))
-    end
-    return U, s, Vt
+    return svd_into!(U, s, Vt, A)
 end
 
-Muscle.choose_backend_rule(::typeof(Muscle.tensor_qr_thin), ::DomainROCm) = BackendHIP()
+# In place (src/Operations/tensor_svd.jl:129): the factors are written straight
+# into the caller's U, s, V (the bond is s's index; U's other indices are the
+# row indices), with no intermediate copy.
+function Muscle.tensor_svd_thin!(::BackendHIP, U::Tensor, s::Tensor, V::Tensor, A::Tensor; kwargs...)
+    ind_s = only(inds(s))
+    inds_u, inds_v = setdiff(inds(U), [ind_s]), setdiff(inds(V), [ind_s])
+    issetequal(inds_u ∪ inds_v, inds(A)) || throw(ArgumentError("inds(U) and inds(V) must cover inds(A)"))
+    size(s, ind_s) == min(prod(i -> size(A, i), inds_u), prod(i -> size(A, i), inds_v)) ||
+        throw(DimensionMismatch("s must hold min(rows, columns) singular values"))
+    return svd_into!(U, s, V, A)
+end
+
+# the cut of src/Operations/tensor_svd.jl:183-203, on the host: at most maxdim
+# values, then up to the first one below threshold * norm(s)
+function trunc_dim(s::Tensor, threshold, maxdim)
+    sh = Array(parent(s))
+    k = length(sh)
+    isnothing(maxdim) || (k = min(k, maxdim))
+    isnothing(threshold) || (k = something(findfirst(<(sqrt(sum(abs2, sh)) * threshold), view(sh, 1:k)), k))
+    return k
+end
+
+function Muscle.tensor_svd_trunc(::BackendHIP, A::Tensor; inds_u=(), inds_v=(), ind_s=Index(gensym(:vind)),
+                                 threshold=nothing, maxdim=nothing, kwargs...)
+    U, s, Vt = Muscle.tensor_svd_thin(BackendHIP(), A; inds_u, inds_v, ind_s)
+    k = trunc_dim(s, threshold, maxdim)
+    return view(U, ind_s => 1:k), view(s, ind_s => 1:k), view(Vt, ind_s => 1:k)
+end
+
+# In place (src/Operations/tensor_svd.jl:217): as BackendBase does, the
+# truncated factors must have the caller's sizes and are copied into them.
+function Muscle.tensor_svd_trunc!(::BackendHIP, U::Tensor, s::Tensor, V::Tensor, A::Tensor; threshold=nothing,
+                                  maxdim=nothing, kwargs...)
+    ind_s = only(inds(s))
+    tU, ts, tV = Muscle.tensor_svd_trunc(BackendHIP(), A; inds_u=setdiff(inds(U), [ind_s]),
+                                         inds_v=setdiff(inds(V), [ind_s]), ind_s, threshold, maxdim)
+    inds(tU) == inds(U) && inds(ts) == inds(s) && inds(tV) == inds(V) ||
+        throw(ArgumentError("output indices differ from the factorization's"))
+    size(tU) == size(U) && size(ts) == size(s) && size(tV) == size(V) ||
+        throw(DimensionMismatch("truncated factors have sizes $(size(tU)), $(size(ts)), $(size(tV))"))
+    copyto!(parent(U), parent(tU))
+    copyto!(parent(s), parent(ts))
+    copyto!(parent(V), parent(tV))
+    return U, s, V
+end
+
+# muscle_hip_qr_thin into caller-owned Q[inds_q..., v], R[v, inds_r...]
+function qr_into!(Q::Tensor, R::Tensor, A::Tensor)
+    labels = modemap(Q, R, A)
+    dq, kq = describe(Q, labels)
+    dr, kr = describe(R, labels)
+    da, ka = describe(A, labels)
+    with_handle() do h
+        GC.@preserve Q R A kq kr ka begin
+            check(ccall((:muscle_hip_qr_thin, LIB), Cint, (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}),
+                        h, dq, dr, da))
+        end
+    end
+    return Q, R
+end
 
 # Q[inds_q..., ind_virtual], R[ind_virtual, inds_r...] with A = Q R
 # (src/Operations/tensor_qr.jl:45-68)
@@ -223,55 +344,16 @@ function Muscle.tensor_qr_thin(::BackendHIP, A::Tensor; inds_q=(), inds_r=(), in
     k = min(prod(sq), prod(sr))
     Q = Tensor(similar(parent(A), sq..., k), [inds_q; ind_virtual])
     R = Tensor(similar(parent(A), k, sr...), [ind_virtual; inds_r])
-    labels = modemap(Q, R, A)
-    dq, kq = describe(Q, labels)
-    dr, kr = describe(R, labels)
-    da, ka = describe(A, labels)
-    GC.@preserve Q R A kq kr ka begin
-        check(ccall((:muscle_hip_qr_thin, LIB), Cint, (Ptr{Cvoid}, Ref{CTensor}, Ref{CTensor}, Ref{CTensor}),
-                    handle(), dq, dr, da))
-    end
-    return Q, R
+    return qr_into!(Q, R, A)
 end
 
-# the thin SVD, then the cut of src/Operations/tensor_svd.jl:183-203 on the host
-function Muscle.tensor_svd_trunc(::BackendHIP, A::Tensor; inds_u=(), inds_v=(), ind_s=Index(gensym(:vind)),
-                                 threshold=nothing, maxdim=nothing, kwargs...)
-    U, s, Vt = Muscle.tensor_svd_thin(BackendHIP(), A; inds_u, inds_v, ind_s)
-    sh = Array(parent(s))
-    k = length(sh)
-    isnothing(maxdim) || (k = min(k, maxdim))
-    isnothing(threshold) || (k = something(findfirst(<(sqrt(sum(abs2, sh)) * threshold), view(sh, 1:k)), k))
-    return view(U, ind_s => 1:k), view(s, ind_s => 1:k), view(Vt, ind_s => 1:k)
-end
-
-# simple_update.jl:37-87 on device tensors: the contractions and the absorption
-# run through this backend's binary_einsum and hadamard!, the SVD through
-# muscle_hip_svd_thin
-function Muscle.simple_update(::BackendHIP, A::Tensor, ind_physical_a::Index, B::Tensor, ind_physical_b::Index,
-                              ind_bond_ab::Index, G::Tensor, ind_physical_g_a::Index, ind_physical_g_b::Index;
-                              normalize::Bool=false, absorb::Muscle.AbsorbBehavior=Muscle.DontAbsorb(),
-                              atol::Float64=0.0, rtol::Float64=0.0, maxdim=nothing)
-    Θ = Muscle.binary_einsum(Muscle.binary_einsum(A, B; dims=[ind_bond_ab]), G; dims=[ind_physical_a, ind_physical_b])
-    Θ = replace(Θ, ind_physical_g_a => ind_physical_a, ind_physical_g_b => ind_physical_b)
-    U, S, V = Muscle.tensor_svd_thin(BackendHIP(), Θ; inds_u=setdiff(inds(A), [ind_bond_ab]),
-                                     inds_v=setdiff(inds(B), [ind_bond_ab]), ind_s=ind_bond_ab)
-    if !isnothing(maxdim)
-        χ = min(maxdim, length(S))
-        U, S, V = view(U, ind_bond_ab => 1:χ), view(S, ind_bond_ab => 1:χ), view(V, ind_bond_ab => 1:χ)
-    end
-    normalize && (parent(S) ./= sqrt(sum(abs2, parent(S))))
-    absorb isa Muscle.DontAbsorb && return U, S, V
-    if absorb isa Muscle.AbsorbU
-        U = Muscle.hadamard!(U, U, S)
-    elseif absorb isa Muscle.AbsorbV
-        V = Muscle.hadamard!(V, V, S)
-    elseif absorb isa Muscle.AbsorbEqually
-        S_sqrt = Tensor(sqrt.(parent(S)), inds(S))
-        U = Muscle.hadamard!(U, U, S_sqrt)
-        V = Muscle.hadamard!(V, V, S_sqrt)
-    end
-    return U, V
+# In place (src/Operations/tensor_qr.jl:70): written straight into the caller's
+# Q and R; the bond is the index they share.
+function Muscle.tensor_qr_thin!(::BackendHIP, Q::Tensor, R::Tensor, A::Tensor; kwargs...)
+    bond = only(intersect(inds(Q), inds(R)))
+    issetequal(setdiff(inds(Q), [bond]) ∪ setdiff(inds(R), [bond]), inds(A)) ||
+        throw(ArgumentError("inds(Q) and inds(R) must cover inds(A)"))
+    return qr_into!(Q, R, A)
 end
 
 end # module
