@@ -35,7 +35,24 @@ for _p in (ROOT, os.path.join(ROOT, "muscle.jl_amd")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0, "int8": 5000.0}
+
+
+def issued_mfma(precision: str, complex_mult: int, slices: int, mnkb: float, kernel: int = 0):
+    """MFMA work a complex x complex contraction of M*N*K*batch = mnkb index points
+    issues: (real flops or int ops, the MFMA type's dense peak in T/s).  Native and
+    FP32 issue 3 (3M) or 4 (4M) real products of 2*mnkb; the sliced rungs issue
+    their slice pairs on each of the 3 planes (re, im, re+im): BF16x3 6 pairs,
+    FP16x2 3, Ozaki int8 S(S+1)/2 (every pair of weight i + j <= S + 1); the
+    modular Ozaki form (kernel 7) one product per modulus and plane."""
+    if precision in ("native", "fp32"):
+        prods = 3 if complex_mult == 3 else 4
+        return prods * 2.0 * mnkb, PEAK_TFLOPS["fp64" if precision == "native" else "fp32"]
+    if kernel == 7:
+        return 3 * slices * 2.0 * mnkb, PEAK_TFLOPS["int8"]
+    pairs = {"bf16x3": 6, "fp16x2": 3}.get(precision, slices * (slices + 1) // 2)
+    peak = {"bf16x3": "bf16", "fp16x2": "fp16"}.get(precision, "int8")
+    return 3 * pairs * 2.0 * mnkb, PEAK_TFLOPS[peak]
 HBM_TBS = 8.0
 METRIC = "binary_einsum TFLOP/s (% MFMA peak) vs plain at stated error; jobs/s + p99 latency"
 
@@ -190,9 +207,12 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
             bt.run(mode)
     ms_p, _ = timed(lambda: run_all("persistent"), steps, 1)
     last_c = [c.parent().clone() for c, a, b in wl.rounds[-1]]
-    ms_l, _ = timed(lambda: run_all("loop"), loop_steps, 1)
-    diff = max(float(torch.linalg.vector_norm(c.parent() - x) / torch.linalg.vector_norm(x).clamp_min(1e-300))
-               for (c, a, b), x in zip(wl.rounds[-1], last_c))
+    if loop_steps > 0:
+        ms_l, _ = timed(lambda: run_all("loop"), loop_steps, 1)
+        diff = max(float(torch.linalg.vector_norm(c.parent() - x) / torch.linalg.vector_norm(x).clamp_min(1e-300))
+                   for (c, a, b), x in zip(wl.rounds[-1], last_c))
+    else:
+        ms_l, diff = float("nan"), float("nan")
     del last_c
     out = {
         "dtype": dtype, "pairs": n, "rounds": len(wl.rounds), "output_budget_gib": budget_gb,
@@ -524,6 +544,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=10000)
     ap.add_argument("--batch-budget-gib", type=float, default=96.0)
+    ap.add_argument("--batch-dtypes", default="c64,c32", help="element types of the batched workload")
+    ap.add_argument("--batch-loop-steps", type=int, default=1, help="timed runs of the plain per-call loop (0: none)")
     ap.add_argument("--served-jobs", type=int, default=32, help="jobs per rate, full-size trees")
     ap.add_argument("--served-small-jobs", type=int, default=200, help="jobs per rate, small trees")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -560,6 +582,7 @@ def main() -> int:
     ms, c, info = contraction_case(a, b, {"out": ic}, head_be, args.steps, args.warmup)
     flops = info["flops"]
     tflops = flops / (ms * 1e-3) / 1e12
+    head_ops, head_peak = issued_mfma(args.precision, args.complex_mult, info["slices"], flops / 8.0, info["kernel"])
     log(f"headline {ms:.3f} ms/step, {tflops:.2f} TFLOP/s")
     cpu, ref_c2 = None, None
     if not args.no_cpu_baseline:
@@ -594,16 +617,21 @@ def main() -> int:
             be = muscle.BackendHIP(precision=mode, target_rel_error=tgt or 1e-12, complex_mult=args.complex_mult)
             m2, c2, inf2 = contraction_case(a, b, {"out": ic}, be, max(3, args.steps // 2), 1)
             key = mode if mode != "ozaki_int8" else f"ozaki_int8_{tgt:g}"
+            ops, pk = issued_mfma(mode, args.complex_mult, inf2["slices"], flops / 8.0, inf2["kernel"])
             lad[key] = {"ms": round(m2, 4), "tflops": round(flops / (m2 * 1e-3) / 1e12, 3),
                         "speedup_vs_plain": round(plain_ms / m2, 3), "slices": inf2["slices"],
-                        "err_model": inf2["err_bound"]}
+                        "kernel": inf2["kernel"],
+                        "err_model": inf2["err_bound"],
+                        "issued_mfma_tops_per_call": round(ops / 1e12, 3),
+                        "mfma_util": round(ops / (m2 * 1e-3) / 1e12 / pk, 4)}
             if ref_c2 is not None:
                 lad[key]["rel_err_vs_cpu_oracle"] = rel(c2.parent().cpu().numpy(), ref_c2)
             log(f"ladder {key}: {lad[key]}")
             del c2
         m4, c4m, _ = contraction_case(a, b, {"out": ic}, muscle.BackendHIP(complex_mult=4), max(3, args.steps // 2),
                                       1)
-        lad["plain_4m"] = {"ms": round(m4, 4), "tflops": round(flops / (m4 * 1e-3) / 1e12, 3)}
+        lad["plain_4m"] = {"ms": round(m4, 4), "tflops": round(flops / (m4 * 1e-3) / 1e12, 3),
+                           "mfma_util": round(flops / (m4 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"], 4)}
         if ref_c2 is not None:
             lad["plain_4m"]["rel_err_vs_cpu_oracle"] = rel(c4m.parent().cpu().numpy(), ref_c2)
         del c4m
@@ -687,9 +715,10 @@ def main() -> int:
             res[dist] = row
         extra["ladder_c1"] = res
     if want("batched"):
-        for bdt in ("c64", "c32"):
+        for bdt in args.batch_dtypes.split(","):
             try:
-                extra[f"batched_{bdt}"] = bench_batched(bdt, args.batch_pairs, 0, args.batch_budget_gib, 3, 1,
+                extra[f"batched_{bdt}"] = bench_batched(bdt, args.batch_pairs, 0, args.batch_budget_gib, 3,
+                                                        args.batch_loop_steps,
                                                         0 if args.no_cpu_baseline else 300, threads)
                 log(f"batched {bdt}: {extra[f'batched_{bdt}']}")
             except Exception as e:  # reported in the JSON, not hidden
@@ -742,7 +771,11 @@ def main() -> int:
                    "precision": args.precision, "complex_mult": args.complex_mult, "m": info["m"], "n": info["n"],
                    "k": info["k"], "batch": info["batch"], "kernel": info["kernel"], "flops_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(tflops / peak, 4), "traffic": measured_traffic(args.precision)},
+                     "frac": round(tflops / peak, 4), "traffic": measured_traffic(args.precision),
+                     "issued_mfma_tflops": round(head_ops / (ms * 1e-3) / 1e12, 3),
+                     "mfma_peak": head_peak, "mfma_util": round(head_ops / (ms * 1e-3) / 1e12 / head_peak, 4),
+                     "note": "achieved and frac count 8*M*N*K algorithmic flops; mfma_util counts the MFMA work "
+                             "the kernel issues (3M: 6*M*N*K) over that MFMA type's dense peak"},
         "cpu_baseline": cpu,
         "extra": extra,
     }

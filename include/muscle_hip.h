@@ -73,9 +73,17 @@ typedef enum {
  *               K-chunk, chunks summed in FP64.
  *   FP16X2      two float16 terms, three products, FP32 accumulation per
  *               K-chunk, chunks summed in FP64.
- *   OZAKI_INT8  error-free int8 slicing of power-of-two-scaled rows of A and
- *               columns of B, exact int32 MFMA accumulation, FP64
- *               recombination; the slice count comes from target_rel_error.
+ *   OZAKI_INT8  exact int32 MFMA products of int8 data.  Large contractions
+ *               (M*N >= 512^2, M*N*K*batch >= 1e9, K <= 2^17 - 128) scale
+ *               each row of A and column of B to a 2-norm of 2^T, round to
+ *               integers and multiply them modulo up to 13 pairwise coprime
+ *               moduli <= 256 (Ozaki scheme II), recombining each output by
+ *               the Chinese remainder theorem in FP64; the modulus count is
+ *               the smallest whose error model sqrt(K/3) 2^-T meets
+ *               target_rel_error.  Other shapes (or targets beyond 13 moduli)
+ *               slice power-of-two-scaled rows into balanced base-255 int8
+ *               digits (Ozaki scheme I) with FP64 recombination; the slice
+ *               count comes from target_rel_error.
  */
 typedef enum {
   MUSCLE_HIP_PRECISION_NATIVE = 0,
@@ -99,9 +107,11 @@ typedef struct {
 typedef struct {
   int32_t kernel;        /* 0 none, 1 MFMA tile GEMM, 2 reduction, 3 split-precision, 4 elementwise,
                             5 split-precision with the int8 slice products as hipBLASLt GEMMs,
-                            6 tiled permutation (unary_einsum with no summed modes, transposed through LDS) */
+                            6 tiled permutation (unary_einsum with no summed modes, transposed through LDS),
+                            7 Ozaki int8 in modular form: int8 MFMA products modulo small moduli,
+                              recombined by the Chinese remainder theorem */
   int32_t precision;     /* muscle_hip_precision_t actually used */
-  int32_t slices;        /* Ozaki slices per operand (0 otherwise) */
+  int32_t slices;        /* Ozaki slices per operand, or moduli of the modular form (0 otherwise) */
   int32_t swapped;       /* 1 if A and B were exchanged to make C's fastest mode an N mode */
   int64_t m, n, k, batch;/* fused GEMM extents */
   double flops;          /* algorithmic real flops (8*M*N*K*batch for complex) */

@@ -16,6 +16,7 @@
 #include "handle.hpp"
 #include "kernels.hpp"
 #include "lowering.hpp"
+#include "crt.hpp"
 #include "split.hpp"
 
 namespace mh {
@@ -387,7 +388,7 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
   const ContractionDesc& d = p.d;
   info->kernel = p.kind == Plan::GEMM      ? 1
                  : p.kind == Plan::REDUCE  ? 2
-                 : p.kind == Plan::SPLIT   ? (p.split_lib ? 5 : 3)
+                 : p.kind == Plan::SPLIT   ? (p.split_mode == kSplitModeCrt ? 7 : p.split_lib ? 5 : 3)
                  : p.kind == Plan::PERMUTE ? 6
                                            : 0;
   info->precision = p.fp32_math ? MUSCLE_HIP_PRECISION_FP32

@@ -1,0 +1,28 @@
+// Ozaki scheme II: the Ozaki int8 rung of large contractions as exact int8
+// GEMMs modulo small moduli, recombined by the Chinese remainder theorem.
+#pragma once
+
+#include "handle.hpp"
+
+namespace mh {
+
+constexpr int kCrtMaxModuli = 16;
+// int32 sums of Kp <= kCrtMaxK - 128 products of residues (|r| <= 128) are exact
+constexpr int64_t kCrtMaxK = int64_t(1) << 17;
+// Plan::split_mode of a CRT plan (the slicing modes are 0..2)
+constexpr int kSplitModeCrt = 3;
+
+// Bits T of the scaled operands' row norms with n moduli (the CRT range
+// M / 2 exceeds (2^T + rounding)^2), and the error model sqrt(K / 3) 2^-T.
+int crt_bits(int n, int64_t K);
+double crt_model(int n, int64_t K);
+// Largest modulus count whose scaled integers stay below 2^50 (exact in FP64).
+int crt_max_moduli(int64_t K);
+// Shapes the CRT path takes: M * N >= 512^2, M * N * K * batch >= 1e9, K <= 2^17 - 128.
+bool crt_eligible(const ContractionDesc& d);
+// Sets the modulus count for the target, the padded extents and err_bound.
+void prepare_crt(Plan& p, double target);
+size_t crt_workspace_bytes(const Plan& p);
+void run_crt(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca, bool cb, void* ws);
+
+}  // namespace mh

@@ -31,6 +31,7 @@
 #include <cstring>
 #include <vector>
 
+#include "crt.hpp"
 #include "device_types.hpp"
 #include "kernels.hpp"
 #include "split.hpp"
@@ -597,6 +598,13 @@ void prepare_split(Plan& p, double target) {
   int64_t kchunk = d.K;
   switch (p.precision) {
     case MUSCLE_HIP_PRECISION_OZAKI_INT8: {
+      // large shapes: products modulo small moduli (fewer int8 GEMMs than
+      // slicing), when the modulus count the target needs keeps the scaled
+      // integers exact in FP64
+      if (crt_eligible(d) && crt_model(crt_max_moduli(d.K), d.K) <= target) {
+        prepare_crt(p, target);
+        return;
+      }
       p.split_mode = MODE_I8;
       p.esize = 1;
       int S = 3;
@@ -685,11 +693,12 @@ void prepare_split(Plan& p, double target) {
 
 size_t split_workspace_bytes(const Plan& p) {
   if (p.kind != Plan::SPLIT) return 0;
+  if (p.split_mode == kSplitModeCrt) return crt_workspace_bytes(p);
   return ws_layout(p).total;
 }
 
 void upload_split_tables(Plan& p, hipStream_t s) {
-  if (p.kind != Plan::SPLIT) return;
+  if (p.kind != Plan::SPLIT || p.split_mode == kSplitModeCrt) return;
   const size_t nseg = p.split_segs.size() * 4, ngrp = p.split_groups.size() * 4;
   p.dev_groups_off = (nseg + 255) & ~size_t(255);
   p.dev_coef_off = p.dev_groups_off + ((ngrp + 255) & ~size_t(255));
@@ -905,6 +914,10 @@ bool run_lib_groups(Handle& h, Plan& p, const SlicedArgs& ga) {
 }
 
 void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca, bool cb, void* ws) {
+  if (p.split_mode == kSplitModeCrt) {
+    run_crt(h, p, C, A, B, ca, cb, ws);
+    return;
+  }
   const ContractionDesc& d = p.d;
   const WsLayout w = ws_layout(p);
   char* base = (char*)ws;

@@ -63,26 +63,70 @@ def test_ozaki_int8(shape, T, target, S):
     assert rel_err(C, ref) <= target, rel_err(C, ref)
 
 
-# large enough (M*N*K*batch >= 1e9) for the slice products to run as library
-# GEMMs; ragged extents, a batch (hyper) mode and a second call on the tuned plan
-LIB_SHAPES = [
+# Large shapes (M*N >= 512^2, M*N*K*batch >= 1e9, K < 2^17) take the modular
+# (CRT) form of the Ozaki rung: int8 products modulo 10-13 small moduli on this
+# library's int8 MFMA GEMM, recombined by the Chinese remainder theorem.
+# Ragged extents (padding to the 256-wide tiles), a batch (hyper) mode, an
+# M-contiguous and a K-contiguous operand, and a second call on the cached plan.
+CRT_SHAPES = [
     ("square 1024", ["i", "j"], (1024, 1024), ["j", "k"], (1024, 1024), ["i", "k"]),
     ("ragged", ["i", "j"], (1000, 1000), ["j", "k"], (1000, 1100), ["k", "i"]),
     ("hyper", ["i", "j", "h"], (1024, 512, 2), ["j", "k", "h"], (512, 1024, 2), ["h", "k", "i"]),
+    ("k-major A", ["j", "i"], (1100, 1030), ["j", "k"], (1100, 990), ["i", "k"]),
 ]
 
 
-@pytest.mark.parametrize("target, S", [(1e-10, 5), (1e-12, 6)])
-@pytest.mark.parametrize("shape", LIB_SHAPES, ids=lambda s: s[0])
-def test_ozaki_int8_library_groups(shape, target, S):
+@pytest.mark.parametrize("target", [1e-10, 1e-12])
+@pytest.mark.parametrize("shape", CRT_SHAPES, ids=lambda s: s[0])
+def test_ozaki_int8_crt(shape, target):
     name, ia, sa, ib, sb, ic = shape
     rng = np.random.default_rng(8)
     A, B = randn(rng, sa, np.complex128), randn(rng, sb, np.complex128)
     ref = bb.binary_einsum_general(ic, A, ia, B, ib)
-    for call in range(2):  # the first call tunes the library algorithms
+    first = None
+    for call in range(2):
         C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=target)
-        assert info["kernel"] == 5 and info["slices"] == S, info
+        assert info["kernel"] == 7 and 10 <= info["slices"] <= 13, info
+        assert info["err_bound"] <= target
         assert rel_err(C, ref) <= target, (call, rel_err(C, ref))
+        if first is None:
+            first = C
+        else:
+            assert np.array_equal(first, C)  # deterministic
+
+
+@pytest.mark.parametrize("ta, tb", [(np.float64, np.float64), (np.float64, np.complex128),
+                                    (np.complex128, np.float64)])
+def test_ozaki_int8_crt_real_and_mixed(ta, tb):
+    rng = np.random.default_rng(9)
+    A, B = randn(rng, (1030, 1000), ta), randn(rng, (1000, 1040), tb)
+    C, info = run(A, ["i", "j"], B, ["j", "k"], ["i", "k"], precision="ozaki_int8", target_rel_error=1e-10)
+    assert info["kernel"] == 7
+    assert rel_err(C, A @ B) <= 1e-10
+
+
+@pytest.mark.parametrize("dist", ["loguniform", "unitrows"])
+def test_ozaki_int8_crt_distributions(dist):
+    """The log-uniform (1e-4..1e4) and unit-row distributions of the ladder at a
+    CRT-sized shape, with an all-zero row of A and column of B (their C row and
+    column must be exactly zero)."""
+    rng = np.random.default_rng(10)
+    n = 1024
+    if dist == "loguniform":
+        A = (10.0 ** rng.uniform(-4, 4, (n, n))) * np.exp(2j * np.pi * rng.uniform(size=(n, n)))
+        B = (10.0 ** rng.uniform(-4, 4, (n, n))) * np.exp(2j * np.pi * rng.uniform(size=(n, n)))
+    else:
+        A, B = randn(rng, (n, n)), randn(rng, (n, n))
+        A /= np.linalg.norm(A, axis=1, keepdims=True)
+        B /= np.linalg.norm(B, axis=0, keepdims=True)
+    A[5, :] = 0
+    B[:, 7] = 0
+    ref = A @ B
+    for target in (1e-10, 1e-12):
+        C, info = run(A, ["i", "j"], B, ["j", "k"], ["i", "k"], precision="ozaki_int8", target_rel_error=target)
+        assert info["kernel"] == 7
+        assert rel_err(C, ref) <= target, (dist, target, rel_err(C, ref))
+        assert not C[5, :].any() and not C[:, 7].any()
 
 
 @pytest.mark.parametrize("ta, tb", [(np.float64, np.complex128), (np.complex128, np.float64)])

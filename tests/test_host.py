@@ -112,6 +112,30 @@ def test_ozaki_slice_count_from_target(target, S):
     assert d["err_bound"] <= target
 
 
+@pytest.mark.parametrize("target, n", [(1e-8, 9), (1e-10, 10), (1e-12, 12), (1e-13, 13)])
+def test_ozaki_crt_modulus_count_from_target(target, n):
+    """Large shapes take the modular (CRT) form: the smallest modulus count n whose
+    error model sqrt(K / 3) 2^-T meets the target, where 2^(2T) stays below half
+    the moduli's product (K = 4096: T = 35, 39, 43, 46, 50 for n = 9..13)."""
+    s = [16] * 6
+    d = describe(["a1", "b1", "a3", "b2", "a5", "b3"], s, ["a1", "c1", "a3", "c2", "a5", "c3"], s,
+                 ["c1", "b1", "c2", "b2", "c3", "b3"], s, precision="ozaki_int8", target_rel_error=target)
+    assert d["kernel_name"] == "split_crt" and d["slices"] == n, d
+    bits = {9: 35, 10: 39, 11: 43, 12: 46, 13: 50}[n]
+    assert d["err_bound"] == pytest.approx((4096 / 3) ** 0.5 * 2.0 ** -bits, rel=1e-12)
+    assert d["err_bound"] <= target
+
+
+def test_ozaki_crt_falls_back_to_slicing():
+    # too small for the CRT GEMM tiles, and a target beyond 13 moduli (2^50 integers)
+    small = (["i", "k"], [64, 96], ["i", "j"], [64, 1536], ["j", "k"], [1536, 96])
+    assert describe(*small, precision="ozaki_int8", target_rel_error=1e-10)["kernel_name"] == "split"
+    big = (["i", "k"], [1024, 1024], ["i", "j"], [1024, 1024], ["j", "k"], [1024, 1024])
+    assert describe(*big, precision="ozaki_int8", target_rel_error=1e-10)["kernel_name"] == "split_crt"
+    d = describe(*big, precision="ozaki_int8", target_rel_error=1e-16)
+    assert d["kernel_name"] in ("split", "split_library") and d["slices"] == 7
+
+
 def test_ladder_modes_lowering():
     for mode in ("fp32", "bf16x3", "fp16x2"):
         d = describe(["i", "k"], [64, 64], ["i", "j"], [64, 128], ["j", "k"], [128, 64], precision=mode)
