@@ -106,7 +106,7 @@ typedef struct {
 /* What the library did for one contraction (optional out-parameter). */
 typedef struct {
   int32_t kernel;        /* 0 none, 1 MFMA tile GEMM, 2 reduction, 3 split-precision, 4 elementwise,
-                            5 split-precision with the int8 slice products as hipBLASLt GEMMs,
+                            5 (unused),
                             6 tiled permutation (unary_einsum with no summed modes, transposed through LDS),
                             7 Ozaki int8 in modular form: int8 MFMA products modulo small moduli,
                               recombined by the Chinese remainder theorem */

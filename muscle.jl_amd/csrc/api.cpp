@@ -80,14 +80,12 @@ void Handle::ensure_workspace(size_t bytes) {
 Plan::~Plan() {
   if (dtab) (void)hipFree(dtab);
   if (split_dev) (void)hipFree(split_dev);
-  release_split_lib(*this);
 }
 
 Handle::~Handle() {
   if (order_event) (void)hipEventDestroy(order_event);
   if (ws) (void)hipFree(ws);
   cache.clear();  // plans free their device tables
-  release_lt(*this);
   release_rb(*this);
 }
 
@@ -388,7 +386,7 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
   const ContractionDesc& d = p.d;
   info->kernel = p.kind == Plan::GEMM      ? 1
                  : p.kind == Plan::REDUCE  ? 2
-                 : p.kind == Plan::SPLIT   ? (p.split_mode == kSplitModeCrt ? 7 : p.split_lib ? 5 : 3)
+                 : p.kind == Plan::SPLIT   ? (p.split_mode == kSplitModeCrt ? 7 : 3)
                  : p.kind == Plan::PERMUTE ? 6
                                            : 0;
   info->precision = p.fp32_math ? MUSCLE_HIP_PRECISION_FP32

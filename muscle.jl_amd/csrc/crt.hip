@@ -432,11 +432,14 @@ __global__ __launch_bounds__(GNT, 1) void crt_gemm_kernel(CrtGemmArgs p) {
       const int cur = ks & 1;
       if (ks + 1 < GKB / 32) read_frags(cur ^ 1, buf, ks + 1);
       __builtin_amdgcn_sched_barrier(0);  // the next k-step's reads stay ahead of these MFMAs
+      // the MFMAs issue at raised priority, ahead of the other wave's copies and reads
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][i], fb[cur][j], acc[i][j], 0, 0, 0);
+      __builtin_amdgcn_s_setprio(0);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -496,6 +499,7 @@ __device__ __forceinline__ float mod_pos(int c, int y, int p, float invp_f) {
 // residue plane).  x / M = sum_i ((c_i y_i) mod p_i) / p_i (mod 1); its
 // centred fractional part times M is the exact integer result up to FP64
 // rounding of the fraction (about N * 2^-53 absolute).
+template <int NMOD>  // moduli in use (p.n), fixed at compile time so the residue loads unroll into registers
 __global__ __launch_bounds__(NT) void crt_combine_kernel(CrtCombineArgs p) {
   const int Mq = p.Mp >> 2;
   const int64_t total = (int64_t)p.B * Mq * p.N;
@@ -505,13 +509,18 @@ __global__ __launch_bounds__(NT) void crt_combine_kernel(CrtCombineArgs p) {
   const int64_t t = o / p.N;
   const int mq = (int)(t % Mq), b = (int)(t / Mq);
   if (4 * mq >= p.M) return;
-  const int64_t plane = (int64_t)p.B * p.Mp * p.Np;
-  const int64_t at = ((int64_t)b * Mq + mq) * p.Np + n;  // word index within a plane
+  const int64_t plane = (int64_t)p.B * p.Mp * p.Np / 4;  // words per residue plane
+  const uint32_t* Rw = reinterpret_cast<const uint32_t*>(p.R) + ((int64_t)b * Mq + mq) * p.Np + n;
+  // every residue word of this thread first, so all the loads are in flight together
+  uint32_t wv[NMOD][3];
+#pragma unroll
+  for (int i = 0; i < NMOD; ++i)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) wv[i][q] = q < p.nprod ? Rw[(int64_t)(i * p.nprod + q) * plane] : 0u;
   double fre[4] = {0.0, 0.0, 0.0, 0.0}, fim[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = 0; i < p.n; ++i) {
-    uint32_t w[3] = {0u, 0u, 0u};
-    for (int q = 0; q < p.nprod; ++q)
-      w[q] = reinterpret_cast<const uint32_t*>(p.R + (int64_t)(i * p.nprod + q) * plane)[at];
+#pragma unroll
+  for (int i = 0; i < NMOD; ++i) {
+    const uint32_t* w = wv[i];
     const int pi = p.p[i], yi = p.y[i];
     const float ipf = p.invp_f[i];
     const double ipd = p.invp[i];
@@ -742,7 +751,22 @@ void run_crt(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca,
   }
   cm.Mval = Mv;
   const int64_t total = d.B * (p.Mp / 4) * d.N;
-  hipLaunchKernelGGL(crt_combine_kernel, dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0, s, cm);
+  const dim3 cgrid((unsigned)((total + NT - 1) / NT));
+  switch (n) {
+    case 2: hipLaunchKernelGGL(crt_combine_kernel<2>, cgrid, dim3(NT), 0, s, cm); break;
+    case 3: hipLaunchKernelGGL(crt_combine_kernel<3>, cgrid, dim3(NT), 0, s, cm); break;
+    case 4: hipLaunchKernelGGL(crt_combine_kernel<4>, cgrid, dim3(NT), 0, s, cm); break;
+    case 5: hipLaunchKernelGGL(crt_combine_kernel<5>, cgrid, dim3(NT), 0, s, cm); break;
+    case 6: hipLaunchKernelGGL(crt_combine_kernel<6>, cgrid, dim3(NT), 0, s, cm); break;
+    case 7: hipLaunchKernelGGL(crt_combine_kernel<7>, cgrid, dim3(NT), 0, s, cm); break;
+    case 8: hipLaunchKernelGGL(crt_combine_kernel<8>, cgrid, dim3(NT), 0, s, cm); break;
+    case 9: hipLaunchKernelGGL(crt_combine_kernel<9>, cgrid, dim3(NT), 0, s, cm); break;
+    case 10: hipLaunchKernelGGL(crt_combine_kernel<10>, cgrid, dim3(NT), 0, s, cm); break;
+    case 11: hipLaunchKernelGGL(crt_combine_kernel<11>, cgrid, dim3(NT), 0, s, cm); break;
+    case 12: hipLaunchKernelGGL(crt_combine_kernel<12>, cgrid, dim3(NT), 0, s, cm); break;
+    case 13: hipLaunchKernelGGL(crt_combine_kernel<13>, cgrid, dim3(NT), 0, s, cm); break;
+    default: fail(MUSCLE_HIP_INVALID_ARGUMENT, "internal: CRT modulus count out of range");
+  }
 }
 
 }  // namespace mh

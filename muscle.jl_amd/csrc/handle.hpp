@@ -67,11 +67,6 @@ struct Plan {
   std::vector<double> split_coef;     // [2][ngroups]: real / imaginary combine weights
   void* split_dev = nullptr;          // device copy: segs | groups | coef
   size_t dev_groups_off = 0, dev_coef_off = 0;
-  // int8 slice products as library GEMMs (one per weight group and complex
-  // product); split_lib says the plan qualifies, lt the per-group state
-  bool split_lib = false;
-  std::vector<int32_t> lib_groups;  // per output group: a plane, b plane, weight d
-  struct LtGroups* lt = nullptr;
   double err_bound = 0.0;
   std::list<std::string>::iterator lru_pos;
 
@@ -103,8 +98,6 @@ struct Handle {
   void* ws = nullptr;
   size_t ws_size = 0;
   int complex_mult = 3;  // MUSCLE_HIP_OPTION_COMPLEX_MULT
-  void* lt = nullptr;     // hipBLASLt handle (created on first use)
-  void* lt_ws = nullptr;  // its workspace (kLtWorkspace bytes)
   void* rb = nullptr;     // rocBLAS handle of the factorizations (created on first use)
   void* fws = nullptr;    // factorization workspace (matrix copies), grown like ws
   size_t fws_size = 0;

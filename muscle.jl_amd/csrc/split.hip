@@ -15,8 +15,8 @@
 //               sums are exact while K * pairs * 127^2 < 2^31.  A packed row
 //               holds the S slices of one operand row side by side (B's in
 //               reverse order), so the pairs of weight d, (i, d - i), are ONE
-//               contiguous K window of A times one of B: large int8 plans run
-//               each group as a plain strided-batched library GEMM.
+//               contiguous K window of A times one of B.  (Large int8 plans
+//               take the modular form of crt.hip instead.)
 //   4. combine: C = 2^(eA + eB) * sum over groups of weight * group sum, in
 //               FP64, scattered to C's own strided layout.
 //
@@ -25,8 +25,6 @@
 // emulation (2024); digits are balanced base-255 (-127..127) so each slice
 // carries log2(255) bits.  BF16x3 / FP16x2 are the hi/mid/lo splits of
 // Henry, Tang & Heinecke (2019).
-#include <hipblaslt/hipblaslt.h>
-
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -660,7 +658,6 @@ void prepare_split(Plan& p, double target) {
   const std::vector<Product> prods = products_for(ca, cb);
   p.split_segs.clear();
   p.split_groups.clear();
-  p.lib_groups.clear();
   std::vector<double> cre, cim;
   // slots: A slice i (1-based) sits at position i - 1 of its row, B slice j
   // at position S - j (reversed)
@@ -676,17 +673,10 @@ void prepare_split(Plan& p, double target) {
           p.split_segs.push_back(k0);
           p.split_segs.push_back(k1);
         }
-        p.lib_groups.push_back(pr.a_plane);
-        p.lib_groups.push_back(pr.b_plane);
-        p.lib_groups.push_back((int32_t)gi + 2);  // weight d: pairs (i, d - i)
         cre.push_back(pr.sre * weights[gi]);
         cim.push_back(pr.sim * weights[gi]);
       }
   p.ngroups = (int)cre.size();
-  // int8 groups as library GEMMs: one K chunk, and large enough that 15-45
-  // launches cost nothing next to the work
-  p.split_lib = p.split_mode == MODE_I8 && nks == 1 && d.M * d.N >= (int64_t)512 * 512 &&
-                (double)d.M * (double)d.N * (double)d.K * (double)d.B >= 1e9;
   p.split_coef = cre;
   p.split_coef.insert(p.split_coef.end(), cim.begin(), cim.end());
 }
@@ -713,204 +703,6 @@ void upload_split_tables(Plan& p, hipStream_t s) {
   MH_HIP_CHECK(hipMemcpyAsync(d + p.dev_groups_off, p.split_groups.data(), ngrp, hipMemcpyHostToDevice, s));
   MH_HIP_CHECK(hipMemcpyAsync(d + p.dev_coef_off, p.split_coef.data(), p.split_coef.size() * 8,
                               hipMemcpyHostToDevice, s));
-}
-
-// ------------------------------------------------------------------ library GEMMs
-// The int8 group sums of large plans are plain dense GEMMs (K-contiguous rows,
-// int32 results), so they run on hipBLASLt: for weight d, the K window of A
-// holding slices 1..d-1 times the K window of B holding slices d-1..1.
-
-constexpr size_t kLtWorkspace = size_t(32) << 20;
-constexpr int kLtCandidates = 16;
-
-struct LtGroups {
-  hipblasLtMatmulDesc_t desc = nullptr;
-  struct Group {
-    hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
-    hipblasLtMatmulAlgo_t cand[kLtCandidates];
-    int ncand = 0, best = -1;
-  };
-  std::vector<Group> g;  // index d - 2
-  // With no hyper (batch) modes, the complex products of one weight d share
-  // a shape and their planes are evenly spaced: one call per d, batched over
-  // the products (nprod > 1), instead of one call per product.
-  int nprod = 1;
-  bool plane_batched = false;
-};
-
-void release_split_lib(Plan& p) {
-  if (!p.lt) return;
-  for (auto& g : p.lt->g) {
-    if (g.la) (void)hipblasLtMatrixLayoutDestroy(g.la);
-    if (g.lb) (void)hipblasLtMatrixLayoutDestroy(g.lb);
-    if (g.lc) (void)hipblasLtMatrixLayoutDestroy(g.lc);
-  }
-  if (p.lt->desc) (void)hipblasLtMatmulDescDestroy(p.lt->desc);
-  delete p.lt;
-  p.lt = nullptr;
-}
-
-void release_lt(Handle& h) {
-  if (h.lt_ws) (void)hipFree(h.lt_ws);
-  if (h.lt) (void)hipblasLtDestroy((hipblasLtHandle_t)h.lt);
-  h.lt_ws = nullptr;
-  h.lt = nullptr;
-}
-
-namespace {
-
-#define MH_LT_OK(x) ((x) == HIPBLAS_STATUS_SUCCESS)
-
-// Builds descriptors, layouts and candidate algorithms; false when the
-// library has no algorithm for some group (the plan then keeps the own kernel).
-bool setup_lt(Handle& h, Plan& p, const SlicedArgs& ga) {
-  if (!h.lt) {
-    hipblasLtHandle_t lt = nullptr;
-    if (!MH_LT_OK(hipblasLtCreate(&lt))) return false;
-    h.lt = lt;
-    if (hipMalloc(&h.lt_ws, kLtWorkspace) != hipSuccess) {
-      (void)hipGetLastError();
-      release_lt(h);
-      return false;
-    }
-  }
-  const ContractionDesc& d = p.d;
-  std::unique_ptr<LtGroups> L(new LtGroups());
-  auto cleanup = [&] {
-    p.lt = L.release();
-    release_split_lib(p);
-    return false;
-  };
-  if (!MH_LT_OK(hipblasLtMatmulDescCreate(&L->desc, HIPBLAS_COMPUTE_32I, HIP_R_32I))) return cleanup();
-  const hipblasOperation_t ta = HIPBLAS_OP_T, tb = HIPBLAS_OP_N;
-  if (!MH_LT_OK(hipblasLtMatmulDescSetAttribute(L->desc, HIPBLASLT_MATMUL_DESC_TRANSA, &ta, sizeof ta)) ||
-      !MH_LT_OK(hipblasLtMatmulDescSetAttribute(L->desc, HIPBLASLT_MATMUL_DESC_TRANSB, &tb, sizeof tb)))
-    return cleanup();
-  hipblasLtMatmulPreference_t pref = nullptr;
-  if (!MH_LT_OK(hipblasLtMatmulPreferenceCreate(&pref))) return cleanup();
-  const uint64_t wsz = kLtWorkspace;
-  (void)hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof wsz);
-  // products: (a plane, b plane) of group entries with weight 2, in order
-  const int ng = (int)(p.lib_groups.size() / 3);
-  std::vector<int> pa, pb;
-  for (int gi = 0; gi < ng; ++gi)
-    if (p.lib_groups[3 * gi + 2] == 2) {
-      pa.push_back(p.lib_groups[3 * gi]);
-      pb.push_back(p.lib_groups[3 * gi + 1]);
-    }
-  L->nprod = (int)pa.size();
-  bool linear = L->nprod > 1;
-  for (int i = 2; i < L->nprod; ++i)
-    linear = linear && pa[i] - pa[i - 1] == pa[1] - pa[0] && pb[i] - pb[i - 1] == pb[1] - pb[0];
-  L->plane_batched = d.B == 1 && linear;
-  const int32_t bc = L->plane_batched ? (int32_t)L->nprod : (int32_t)d.B;
-  const int64_t groups_per_prod = ng / std::max(1, L->nprod);
-  const int64_t sa = L->plane_batched ? (pa.size() > 1 ? (pa[1] - pa[0]) * ga.a_plane_stride : 0) : ga.a_b_stride;
-  const int64_t sb = L->plane_batched ? (pb.size() > 1 ? (pb[1] - pb[0]) * ga.b_plane_stride : 0) : ga.b_b_stride;
-  const int64_t sc = L->plane_batched ? groups_per_prod * p.Mp * p.Np : p.Mp * p.Np;  // int32 elements
-  const int64_t ld = ga.rowbytes;  // int8: bytes == elements
-  bool ok = true;
-  L->g.resize((size_t)p.slices);
-  for (int dd = 2; dd <= p.slices + 1 && ok; ++dd) {
-    LtGroups::Group& G = L->g[(size_t)(dd - 2)];
-    const uint64_t K = (uint64_t)(dd - 1) * (uint64_t)p.Kp;
-    // C^T (N x M, column-major, ld Np) = op(B window)^T (N x K) * A window (K x M)
-    ok = MH_LT_OK(hipblasLtMatrixLayoutCreate(&G.la, HIP_R_8I, K, (uint64_t)d.N, ld)) &&
-         MH_LT_OK(hipblasLtMatrixLayoutCreate(&G.lb, HIP_R_8I, K, (uint64_t)d.M, ld)) &&
-         MH_LT_OK(hipblasLtMatrixLayoutCreate(&G.lc, HIP_R_32I, (uint64_t)d.N, (uint64_t)d.M, p.Np));
-    if (ok && bc > 1) {
-      ok = MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.la, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc)) &&
-           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.la, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &sb,
-                                                      sizeof sb)) &&
-           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lb, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc)) &&
-           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lb, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &sa,
-                                                      sizeof sa)) &&
-           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lc, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc)) &&
-           MH_LT_OK(hipblasLtMatrixLayoutSetAttribute(G.lc, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &sc,
-                                                      sizeof sc));
-    }
-    if (!ok) break;
-    hipblasLtMatmulHeuristicResult_t res[kLtCandidates];
-    int n = 0;
-    ok = MH_LT_OK(hipblasLtMatmulAlgoGetHeuristic((hipblasLtHandle_t)h.lt, L->desc, G.la, G.lb, G.lc, G.lc, pref,
-                                                  kLtCandidates, res, &n)) &&
-         n > 0;
-    for (int i = 0; i < n && ok; ++i) G.cand[i] = res[i].algo;
-    G.ncand = n;
-  }
-  (void)hipblasLtMatmulPreferenceDestroy(pref);
-  if (!ok) return cleanup();
-  p.lt = L.release();
-  return true;
-}
-
-}  // namespace
-
-// Runs the groups as library GEMMs; false leaves them to the own kernel.  The
-// first run of a plan times every candidate algorithm of each group on the
-// plan's own packed operands (synchronizing once) and keeps the fastest.
-bool run_lib_groups(Handle& h, Plan& p, const SlicedArgs& ga) {
-  if (!p.lt && !setup_lt(h, p, ga)) {
-    p.split_lib = false;
-    return false;
-  }
-  hipblasLtHandle_t lt = (hipblasLtHandle_t)h.lt;
-  const int32_t one = 1, zero = 0;
-  const int S = p.slices;
-  const int64_t gstride = (int64_t)p.d.B * p.Mp * p.Np * 4;
-  // in plane-batched mode only the first product's entries are launched
-  const int ng = (int)(p.lib_groups.size() / 3);
-  const int ncall = p.lt->plane_batched ? ng / p.lt->nprod : ng;
-  auto launch = [&](int gi, const hipblasLtMatmulAlgo_t* algo) {
-    const int ap = p.lib_groups[3 * gi], bp = p.lib_groups[3 * gi + 1], dd = p.lib_groups[3 * gi + 2];
-    LtGroups::Group& G = p.lt->g[(size_t)(dd - 2)];
-    const uint8_t* Aw = ga.QA + ap * ga.a_plane_stride;                                      // slices 1..d-1
-    const uint8_t* Bw = ga.QB + bp * ga.b_plane_stride + (int64_t)(S - dd + 1) * ga.slot_bytes;  // d-1..1
-    char* Cg = (char*)ga.out + gi * gstride;
-    return hipblasLtMatmul(lt, p.lt->desc, &one, Bw, G.la, Aw, G.lb, &zero, Cg, G.lc, Cg, G.lc, algo, h.lt_ws,
-                           kLtWorkspace, h.stream);
-  };
-  bool tuned = true;
-  for (auto& G : p.lt->g) tuned = tuned && (G.best >= 0 || G.ncand == 0);
-  if (!tuned) {
-    hipEvent_t e0, e1;
-    MH_HIP_CHECK(hipEventCreate(&e0));
-    MH_HIP_CHECK(hipEventCreate(&e1));
-    for (int gi = 0; gi < ncall; ++gi) {
-      LtGroups::Group& G = p.lt->g[(size_t)(p.lib_groups[3 * gi + 2] - 2)];
-      if (G.best >= 0) continue;
-      float best_ms = 1e30f;
-      for (int c = 0; c < G.ncand; ++c) {
-        if (!MH_LT_OK(launch((int)gi, &G.cand[c]))) continue;  // warm-up; an algorithm that fails is skipped
-        MH_HIP_CHECK(hipEventRecord(e0, h.stream));
-        bool good = true;
-        for (int r = 0; r < 2 && good; ++r) good = MH_LT_OK(launch((int)gi, &G.cand[c]));
-        MH_HIP_CHECK(hipEventRecord(e1, h.stream));
-        MH_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0;
-        MH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (good && ms < best_ms) {
-          best_ms = ms;
-          G.best = c;
-        }
-      }
-      if (G.best < 0) {  // no candidate ran: the own kernel takes the plan
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        release_split_lib(p);
-        p.split_lib = false;
-        return false;
-      }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  }
-  for (int gi = 0; gi < ncall; ++gi) {
-    LtGroups::Group& G = p.lt->g[(size_t)(p.lib_groups[3 * gi + 2] - 2)];
-    if (!MH_LT_OK(launch(gi, &G.cand[G.best])))
-      fail(MUSCLE_HIP_HIP_ERROR, "hipBLASLt int8 GEMM failed");
-  }
-  return true;
 }
 
 void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca, bool cb, void* ws) {
@@ -986,7 +778,7 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
   ga.tiles_m = (int32_t)(p.Mp / SBM);
   ga.tiles_n = (int32_t)(p.Np / SBN);
   ga.esize = p.esize;
-  if (!(p.split_lib && run_lib_groups(h, p, ga))) {
+  {
     dim3 grid((unsigned)(ga.tiles_m * ga.tiles_n), (unsigned)d.B, (unsigned)p.ngroups);
     switch (p.split_mode) {
       case MODE_I8: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_I8>, grid, dim3(NT), 0, s, ga); break;

@@ -26,8 +26,5 @@ size_t split_workspace_bytes(const Plan& p);
 void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca, bool cb, void* ws);
 // Device buffers of a split plan (segments, groups, combine coefficients).
 void upload_split_tables(Plan& p, hipStream_t s);
-// Library GEMM state of a plan / of a handle.
-void release_split_lib(Plan& p);
-void release_lt(Handle& h);
 
 }  // namespace mh
