@@ -44,12 +44,13 @@ def issued_mfma(precision: str, complex_mult: int, slices: int, mnkb: float, ker
     FP32 issue 3 (3M) or 4 (4M) real products of 2*mnkb; the sliced rungs issue
     their slice pairs on each of the 3 planes (re, im, re+im): BF16x3 6 pairs,
     FP16x2 3, Ozaki int8 S(S+1)/2 (every pair of weight i + j <= S + 1); the
-    modular Ozaki form (kernel 7) one product per modulus and plane."""
+    modular Ozaki form one product per modulus and plane (kernel 7: 3 planes,
+    3M; kernel 8: 2 planes, the split form)."""
     if precision in ("native", "fp32"):
         prods = 3 if complex_mult == 3 else 4
         return prods * 2.0 * mnkb, PEAK_TFLOPS["fp64" if precision == "native" else "fp32"]
-    if kernel == 7:
-        return 3 * slices * 2.0 * mnkb, PEAK_TFLOPS["int8"]
+    if kernel in (7, 8):
+        return (3 if kernel == 7 else 2) * slices * 2.0 * mnkb, PEAK_TFLOPS["int8"]
     pairs = {"bf16x3": 6, "fp16x2": 3}.get(precision, slices * (slices + 1) // 2)
     peak = {"bf16x3": "bf16", "fp16x2": "fp16"}.get(precision, "int8")
     return 3 * pairs * 2.0 * mnkb, PEAK_TFLOPS[peak]

@@ -80,7 +80,10 @@ typedef enum {
  *               moduli <= 256 (Ozaki scheme II), recombining each output by
  *               the Chinese remainder theorem in FP64; the modulus count is
  *               the smallest whose error model sqrt(K/3) 2^-T meets
- *               target_rel_error.  Other shapes (or targets beyond 13 moduli)
+ *               target_rel_error.  Complex x complex products run as two
+ *               int8 GEMMs per modulus (moduli with a square root of -1)
+ *               where those moduli reach the target, else as three (3M).
+ *               Other shapes (or targets beyond 13 moduli)
  *               slice power-of-two-scaled rows into balanced base-255 int8
  *               digits (Ozaki scheme I) with FP64 recombination; the slice
  *               count comes from target_rel_error.
@@ -109,7 +112,10 @@ typedef struct {
                             5 (unused),
                             6 tiled permutation (unary_einsum with no summed modes, transposed through LDS),
                             7 Ozaki int8 in modular form: int8 MFMA products modulo small moduli,
-                              recombined by the Chinese remainder theorem */
+                              recombined by the Chinese remainder theorem (complex x complex: 3M,
+                              three products per modulus),
+                            8 as 7, complex x complex products split as Z_p[i] = Z_p x Z_p over
+                              moduli with a square root of -1 (two products per modulus) */
   int32_t precision;     /* muscle_hip_precision_t actually used */
   int32_t slices;        /* Ozaki slices per operand, or moduli of the modular form (0 otherwise) */
   int32_t swapped;       /* 1 if A and B were exchanged to make C's fastest mode an N mode */

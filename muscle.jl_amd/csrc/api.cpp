@@ -386,7 +386,7 @@ void fill_info(muscle_hip_info_t* info, const Plan& p, const muscle_hip_tensor_t
   const ContractionDesc& d = p.d;
   info->kernel = p.kind == Plan::GEMM      ? 1
                  : p.kind == Plan::REDUCE  ? 2
-                 : p.kind == Plan::SPLIT   ? (p.split_mode == kSplitModeCrt ? 7 : 3)
+                 : p.kind == Plan::SPLIT   ? (p.split_mode == kSplitModeCrt ? (p.crt_split ? 8 : 7) : 3)
                  : p.kind == Plan::PERMUTE ? 6
                                            : 0;
   info->precision = p.fp32_math ? MUSCLE_HIP_PRECISION_FP32

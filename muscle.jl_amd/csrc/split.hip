@@ -599,7 +599,7 @@ void prepare_split(Plan& p, double target) {
       // large shapes: products modulo small moduli (fewer int8 GEMMs than
       // slicing), when the modulus count the target needs keeps the scaled
       // integers exact in FP64
-      if (crt_eligible(d) && crt_model(crt_max_moduli(d.K), d.K) <= target) {
+      if (crt_eligible(d) && crt_reaches(d, target)) {
         prepare_crt(p, target);
         return;
       }

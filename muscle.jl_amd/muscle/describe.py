@@ -40,5 +40,5 @@ def describe(inds_c, shape_c, inds_a, shape_a, inds_b, shape_b, dtypes=("c64", "
     out = info.as_dict()
     out["classes"] = {inv[modes[i]]: CLASS_NAMES[classes[i]] for i in range(nm.value)}
     out["kernel_name"] = {0: "none", 1: "gemm_mfma", 2: "reduce", 3: "split", 4: "elementwise", 5: "unused",
-                          6: "permute", 7: "split_crt"}[info.kernel]
+                          6: "permute", 7: "split_crt", 8: "split_crt_gaussian"}[info.kernel]
     return out

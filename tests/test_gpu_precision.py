@@ -64,10 +64,14 @@ def test_ozaki_int8(shape, T, target, S):
 
 
 # Large shapes (M*N >= 512^2, M*N*K*batch >= 1e9, K < 2^17) take the modular
-# (CRT) form of the Ozaki rung: int8 products modulo 10-13 small moduli on this
+# (CRT) form of the Ozaki rung: int8 products modulo 9-13 small moduli on this
 # library's int8 MFMA GEMM, recombined by the Chinese remainder theorem.
+# Complex x complex products split in Z_p x Z_p (kernel 8, two GEMMs per
+# modulus) down to the split table's reach (about 7e-14 at K = 1024); 3e-14
+# needs the 3M form (kernel 7, three GEMMs per modulus).
 # Ragged extents (padding to the 256-wide tiles), a batch (hyper) mode, an
 # M-contiguous and a K-contiguous operand, and a second call on the cached plan.
+CRT_TARGETS = [(1e-10, 8), (1e-12, 8), (3e-14, 7)]
 CRT_SHAPES = [
     ("square 1024", ["i", "j"], (1024, 1024), ["j", "k"], (1024, 1024), ["i", "k"]),
     ("ragged", ["i", "j"], (1000, 1000), ["j", "k"], (1000, 1100), ["k", "i"]),
@@ -76,9 +80,9 @@ CRT_SHAPES = [
 ]
 
 
-@pytest.mark.parametrize("target", [1e-10, 1e-12])
+@pytest.mark.parametrize("target, kernel", CRT_TARGETS)
 @pytest.mark.parametrize("shape", CRT_SHAPES, ids=lambda s: s[0])
-def test_ozaki_int8_crt(shape, target):
+def test_ozaki_int8_crt(shape, target, kernel):
     name, ia, sa, ib, sb, ic = shape
     rng = np.random.default_rng(8)
     A, B = randn(rng, sa, np.complex128), randn(rng, sb, np.complex128)
@@ -86,7 +90,7 @@ def test_ozaki_int8_crt(shape, target):
     first = None
     for call in range(2):
         C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=target)
-        assert info["kernel"] == 7 and 10 <= info["slices"] <= 13, info
+        assert info["kernel"] == kernel and 9 <= info["slices"] <= 13, info
         assert info["err_bound"] <= target
         assert rel_err(C, ref) <= target, (call, rel_err(C, ref))
         if first is None:
@@ -122,11 +126,35 @@ def test_ozaki_int8_crt_distributions(dist):
     A[5, :] = 0
     B[:, 7] = 0
     ref = A @ B
-    for target in (1e-10, 1e-12):
+    for target, kernel in CRT_TARGETS:
         C, info = run(A, ["i", "j"], B, ["j", "k"], ["i", "k"], precision="ozaki_int8", target_rel_error=target)
-        assert info["kernel"] == 7
+        assert info["kernel"] == kernel
         assert rel_err(C, ref) <= target, (dist, target, rel_err(C, ref))
         assert not C[5, :].any() and not C[:, 7].any()
+
+
+@pytest.mark.parametrize("dt, kernel", [(np.complex128, 8), (np.float64, 7)])
+def test_ozaki_int8_crt_nonfinite(dt, kernel):
+    """An Inf in a row of A and a NaN in a column of B (after finite values in
+    that row / column) give NaN across that row / column of C; every other
+    entry keeps the target accuracy."""
+    rng = np.random.default_rng(11)
+    n = 1024
+    A, B = randn(rng, (n, n), dt), randn(rng, (n, n), dt)
+    A[3, 700] = np.inf
+    B[900, 9] = np.nan
+    C, info = run(A, ["i", "j"], B, ["j", "k"], ["i", "k"], precision="ozaki_int8", target_rel_error=1e-10)
+    assert info["kernel"] == kernel
+    assert np.isnan(C[3, :]).all() and np.isnan(C[:, 9]).all()
+    keep_r = np.arange(n) != 3
+    keep_c = np.arange(n) != 9
+    A2, B2 = A.copy(), B.copy()
+    A2[3, 700] = 0
+    B2[900, 9] = 0
+    ref = (A2 @ B2)[np.ix_(keep_r, keep_c)]
+    got = C[np.ix_(keep_r, keep_c)]
+    assert np.isfinite(got).all()
+    assert rel_err(got, ref) <= 1e-10
 
 
 @pytest.mark.parametrize("ta, tb", [(np.float64, np.complex128), (np.complex128, np.float64)])

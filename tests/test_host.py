@@ -112,18 +112,71 @@ def test_ozaki_slice_count_from_target(target, S):
     assert d["err_bound"] <= target
 
 
-@pytest.mark.parametrize("target, n", [(1e-8, 9), (1e-10, 10), (1e-12, 12), (1e-13, 13)])
-def test_ozaki_crt_modulus_count_from_target(target, n):
+# (target, dtype, kernel, moduli n, bits T) at K = 4096.  3M table 256, 255, 253, ...:
+# T = 35, 39, 43, 46, 50 for n = 9..13.  Split table 241, 233, 229, 221, 205, ...
+# (square roots of -1): T = 34, 37, 41, 44, 48 for n = 9..13.
+CRT_COUNTS = [(1e-8, "f64", "split_crt", 9, 35), (1e-10, "f64", "split_crt", 10, 39),
+              (1e-12, "f64", "split_crt", 12, 46), (1e-13, "f64", "split_crt", 13, 50),
+              (1e-8, "c64", "split_crt_gaussian", 9, 34), (1e-10, "c64", "split_crt_gaussian", 11, 41),
+              (1e-12, "c64", "split_crt_gaussian", 13, 48),
+              # beyond the split table's 13 moduli: 3M
+              (1e-13, "c64", "split_crt", 13, 50)]
+
+
+@pytest.mark.parametrize("target, dt, kernel, n, bits", CRT_COUNTS)
+def test_ozaki_crt_modulus_count_from_target(target, dt, kernel, n, bits):
     """Large shapes take the modular (CRT) form: the smallest modulus count n whose
     error model sqrt(K / 3) 2^-T meets the target, where 2^(2T) stays below half
-    the moduli's product (K = 4096: T = 35, 39, 43, 46, 50 for n = 9..13)."""
+    the moduli's product; complex x complex products split in Z_p x Z_p where
+    those moduli reach the target."""
     s = [16] * 6
     d = describe(["a1", "b1", "a3", "b2", "a5", "b3"], s, ["a1", "c1", "a3", "c2", "a5", "c3"], s,
-                 ["c1", "b1", "c2", "b2", "c3", "b3"], s, precision="ozaki_int8", target_rel_error=target)
-    assert d["kernel_name"] == "split_crt" and d["slices"] == n, d
-    bits = {9: 35, 10: 39, 11: 43, 12: 46, 13: 50}[n]
+                 ["c1", "b1", "c2", "b2", "c3", "b3"], s, dtypes=(dt, dt, dt), precision="ozaki_int8",
+                 target_rel_error=target)
+    assert d["kernel_name"] == kernel and d["slices"] == n, d
     assert d["err_bound"] == pytest.approx((4096 / 3) ** 0.5 * 2.0 ** -bits, rel=1e-12)
     assert d["err_bound"] <= target
+
+
+def test_crt_moduli_tables():
+    """The two modulus tables from first principles: pairwise coprime, greedy from
+    256 down, and (split table) every prime factor 1 mod 4, i.e. -1 a square."""
+    import math
+
+    def greedy(ok):
+        ps = []
+        for c in range(256, 64, -1):
+            if ok(c) and all(math.gcd(c, q) == 1 for q in ps):
+                ps.append(c)
+        return ps
+
+    def primes_of(c):
+        f, d = [], 2
+        while d * d <= c:
+            while c % d == 0:
+                f.append(d)
+                c //= d
+            d += 1
+        return f + ([c] if c > 1 else [])
+
+    t3 = greedy(lambda c: True)
+    ts = greedy(lambda c: all(q % 4 == 1 for q in primes_of(c)))
+    assert t3[:6] == [256, 255, 253, 251, 247, 241]
+    assert ts[:6] == [241, 233, 229, 221, 205, 197]
+    for p in ts[:13]:
+        assert any((x * x + 1) % p == 0 for x in range(p)), p
+    # the bit counts the library reports follow from these tables
+    K = 4096
+
+    def bits(ps, n):
+        L = sum(math.log2(p) for p in ps[:n])
+        T = int((L - 1) // 2)
+        while T > 0 and not 2 * math.log2(2.0 ** T + 0.5 * math.sqrt(2 * K)) + 1 < L - 1e-9:
+            T -= 1
+        return T
+
+    assert [bits(t3, n) for n in range(9, 14)] == [35, 39, 43, 46, 50]
+    assert [bits(ts, n) for n in range(9, 14)] == [34, 37, 41, 44, 48]
 
 
 def test_ozaki_crt_falls_back_to_slicing():
@@ -131,7 +184,9 @@ def test_ozaki_crt_falls_back_to_slicing():
     small = (["i", "k"], [64, 96], ["i", "j"], [64, 1536], ["j", "k"], [1536, 96])
     assert describe(*small, precision="ozaki_int8", target_rel_error=1e-10)["kernel_name"] == "split"
     big = (["i", "k"], [1024, 1024], ["i", "j"], [1024, 1024], ["j", "k"], [1024, 1024])
-    assert describe(*big, precision="ozaki_int8", target_rel_error=1e-10)["kernel_name"] == "split_crt"
+    assert describe(*big, precision="ozaki_int8", target_rel_error=1e-10)["kernel_name"] == "split_crt_gaussian"
+    assert describe(*big, dtypes=("f64",) * 3, precision="ozaki_int8",
+                    target_rel_error=1e-10)["kernel_name"] == "split_crt"
     d = describe(*big, precision="ozaki_int8", target_rel_error=1e-16)
     assert d["kernel_name"] == "split" and d["slices"] == 7
 
