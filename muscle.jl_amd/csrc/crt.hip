@@ -352,7 +352,9 @@ __device__ __forceinline__ void digits(double x, uint32_t& lo, uint32_t& hi) {
 // every output plane: CPG planes at a time go through LDS (4 residues of a
 // k-quad per 32-bit word; rows 33 words apart, free of bank conflicts) and
 // leave as 16-byte chunks of K-contiguous rows.
-template <bool CPLX>
+// SPLIT: the planes come in pairs (re + j im, re - j im) of one modulus,
+// which share the re digits' dot products: P+- = R +- I (mod p).
+template <bool CPLX, bool SPLIT>
 __global__ __launch_bounds__(NT) void crt_pack_kernel(CrtPackArgs p) {
   constexpr int RW = CPK / 4 + 1;
   __shared__ uint32_t tile[CPG][CPR][RW];
@@ -390,24 +392,48 @@ __global__ __launch_bounds__(NT) void crt_pack_kernel(CrtPackArgs p) {
   for (int o0 = 0; o0 < p.nout; o0 += CPG) {
     if (o0 > 0) __syncthreads();  // the previous round's tiles have been written out
     const int og = min(CPG, p.nout - o0);
-    for (int g = 0; g < og; ++g) {
-      const PackPlane& c = p.pl[o0 + g];
+    if (SPLIT) {
+      for (int g = 0; g < og; g += 2) {
+        const PackPlane& cp = p.pl[o0 + g];      // re + j im
+        const PackPlane& cm = p.pl[o0 + g + 1];  // re - j im: only its start differs in use
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        uint32_t word = 0;
+        for (int h = 0; h < 2; ++h) {
+          uint32_t wp = 0, wm = 0;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          uint32_t acc = __builtin_amdgcn_udot4(dlo[h][e], c.c[0], c.start, false);
-          acc = __builtin_amdgcn_udot4(dhi[h][e], c.c[1], acc, false);
-          if (CPLX) {
-            acc = __builtin_amdgcn_udot4(ilo[h][e], c.c[2], acc, false);
-            acc = __builtin_amdgcn_udot4(ihi[h][e], c.c[3], acc, false);
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t R = __builtin_amdgcn_udot4(dhi[h][e], cp.c[1], __builtin_amdgcn_udot4(dlo[h][e], cp.c[0], 0u, false), false);
+            const uint32_t I = __builtin_amdgcn_udot4(ihi[h][e], cp.c[3], __builtin_amdgcn_udot4(ilo[h][e], cp.c[2], 0u, false), false);
+            // below 2^20 in magnitude: exact in FP32
+            const float vp = (float)(int32_t)(R + I + cp.start), vm = (float)((int32_t)(R + cm.start) - (int32_t)I);
+            const int32_t rp = (int32_t)fmaf(-rintf(vp * cp.ip), cp.p, vp);
+            const int32_t rm = (int32_t)fmaf(-rintf(vm * cp.ip), cp.p, vm);
+            wp |= ((uint32_t)rp & 0xffu) << (8 * e);
+            wm |= ((uint32_t)rm & 0xffu) << (8 * e);
           }
-          const float v = (float)acc;
-          const int32_t res = (int32_t)fmaf(-rintf(v * c.ip), c.p, v);
-          word |= ((uint32_t)res & 0xffu) << (8 * e);
+          tile[g][rl[h]][kq[h]] = wp;
+          tile[g + 1][rl[h]][kq[h]] = wm;
         }
-        tile[g][rl[h]][kq[h]] = word;
+      }
+    } else {
+      for (int g = 0; g < og; ++g) {
+        const PackPlane& c = p.pl[o0 + g];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          uint32_t word = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            uint32_t acc = __builtin_amdgcn_udot4(dlo[h][e], c.c[0], c.start, false);
+            acc = __builtin_amdgcn_udot4(dhi[h][e], c.c[1], acc, false);
+            if (CPLX) {
+              acc = __builtin_amdgcn_udot4(ilo[h][e], c.c[2], acc, false);
+              acc = __builtin_amdgcn_udot4(ihi[h][e], c.c[3], acc, false);
+            }
+            const float v = (float)acc;
+            const int32_t res = (int32_t)fmaf(-rintf(v * c.ip), c.p, v);
+            word |= ((uint32_t)res & 0xffu) << (8 * e);
+          }
+          tile[g][rl[h]][kq[h]] = word;
+        }
       }
     }
     __syncthreads();
@@ -592,45 +618,44 @@ struct CrtCombineArgs {
   int32_t M, N, B, Mp, Np, nprod, kind;
   int32_t c_complex;
   int32_t n;
-  // re's and im's weights G = y 2^96 / p_i rounded to integers, in six 16-bit
-  // limbs (least significant first): y = y_i = (M / p_i)^-1 mod p_i, times
-  // 1 / 2 and j / 2 (mod p_i) in the split form
-  uint32_t gre[kCrtMaxModuli][6], gim[kCrtMaxModuli][6];
+  // re's and im's weights G = y 2^96 / p_i rounded to integers, as three
+  // 32-bit parts (least significant first) held exactly in FP64:
+  // y = y_i = (M / p_i)^-1 mod p_i, times 1 / 2 and j / 2 (mod p_i) in the
+  // split form
+  double gre[kCrtMaxModuli][3], gim[kCrtMaxModuli][3];
   double Mval;
 };
 
-// Adds c * G (|c| <= 384, G < 2^96 in six 16-bit limbs) to a 96-bit
-// fixed-point fraction kept as six signed limb sums: each product is below
-// 2^24.6 in magnitude, so 13 terms stay below 2^28.3 per limb and the carries
-// wait until the end.
-__device__ __forceinline__ void frac_mac(int32_t (&acc)[6], int32_t c, const uint32_t (&G)[6]) {
+// Adds c * G (|c| <= 384, G < 2^96 as three 32-bit parts g_l) to a 96-bit
+// fixed-point fraction kept as three FP64 part sums: every c g_l is an
+// integer below 2^41 and 13 of them sum below 2^45, so the FMAs are exact and
+// the carries wait until the end.
+__device__ __forceinline__ void frac_mac(double (&acc)[3], double c, const double (&G)[3]) {
 #pragma unroll
-  for (int l = 0; l < 6; ++l) acc[l] += __mul24(c, (int32_t)G[l]);  // both factors fit 24 bits: full-rate multiply
+  for (int l = 0; l < 3; ++l) acc[l] = fma(c, G[l], acc[l]);
 }
 
-// The limb sums' value modulo 2^96 (the fraction modulo 1), centred to
-// [-1/2, 1/2) and rounded to FP64.
-__device__ __forceinline__ double frac_centred(const int32_t (&acc)[6]) {
-  uint32_t w[6];
-  int32_t c = 0;
-#pragma unroll
-  for (int l = 0; l < 6; ++l) {
-    const int32_t t = acc[l] + c;
-    w[l] = (uint32_t)t & 0xffffu;
-    c = t >> 16;  // arithmetic shift: a negative limb sum borrows
-  }
-  const int32_t hi = (int32_t)((w[5] << 16) | w[4]);  // two's complement: the sign of the centred fraction
-  const uint32_t mid = (w[3] << 16) | w[2], lo = (w[1] << 16) | w[0];
-  return ((double)hi + ((double)mid + (double)lo * 0x1p-32) * 0x1p-32) * 0x1p-32;
+// The part sums' value modulo 2^96 (the fraction modulo 1), centred to
+// [-1/2, 1/2) and rounded to FP64: carries move up 32 bits at a time
+// (floor and fma on exact integers), the top part wraps modulo 2^32.
+__device__ __forceinline__ double frac_centred(const double (&acc)[3]) {
+  const double q0 = floor(acc[0] * 0x1p-32), r0 = fma(-q0, 0x1p32, acc[0]);
+  const double t1 = acc[1] + q0;
+  const double q1 = floor(t1 * 0x1p-32), r1 = fma(-q1, 0x1p32, t1);
+  const double t2 = acc[2] + q1;
+  double r2 = fma(-floor(t2 * 0x1p-32), 0x1p32, t2);
+  if (r2 >= 0x1p31) r2 -= 0x1p32;
+  return ((r0 * 0x1p-32 + r1) * 0x1p-32 + r2) * 0x1p-32;
 }
 
 // One thread per 4 consecutive rows of one column (one 32-bit word of every
 // residue plane).  x / M = sum_i c_i y_i / p_i (mod 1) for the residues c_i
 // of re (or im); with G_i = y_i 2^96 / p_i rounded, sum_i c_i G_i (mod 2^96)
-// is that fraction in 96-bit fixed point with wrap-around (the mod 1 is free,
-// no per-modulus reduction is needed, and the rounding of the G_i leaves
-// under 2^-84), so the centred fraction times M is the exact integer result
-// up to one FP64 rounding.
+// is that fraction in 96-bit fixed point (no per-modulus reduction is needed,
+// and the rounding of the G_i leaves under 2^-84).  The sum is kept exactly
+// in three FP64 parts and reduced modulo 2^96 once at the end, so the
+// centred fraction times M is the exact integer result up to one FP64
+// rounding.
 // NMOD (moduli in use) and KIND (0 3M, 1 two products (re, im), 2 one real
 // product, 3 split form) are fixed at compile time, so the residue loads
 // unroll into registers and the loop has no branches.
@@ -638,9 +663,9 @@ template <int NMOD, int KIND>
 __global__ __launch_bounds__(NT) void crt_combine_kernel(CrtCombineArgs p) {
   constexpr int NPROD = KIND == 0 ? 3 : KIND == 2 ? 1 : 2;
   // the weights go through LDS: held in scalar registers they overflow them
-  __shared__ uint32_t sg[NMOD][2][6];
-  if (threadIdx.x < NMOD * 12) {
-    const int i = threadIdx.x / 12, c = (threadIdx.x / 6) & 1, l = threadIdx.x % 6;
+  __shared__ double sg[NMOD][2][3];
+  if (threadIdx.x < NMOD * 6) {
+    const int i = threadIdx.x / 6, c = (threadIdx.x / 3) & 1, l = threadIdx.x % 3;
     sg[i][c][l] = c ? p.gim[i][l] : p.gre[i][l];
   }
   __syncthreads();
@@ -663,7 +688,7 @@ __global__ __launch_bounds__(NT) void crt_combine_kernel(CrtCombineArgs p) {
       wv[i][q] = *Rw;
       Rw += plane;
     }
-  int32_t are[4][6] = {}, aim[4][6] = {};
+  double are[4][3] = {}, aim[4][3] = {};
 #pragma unroll
   for (int i = 0; i < NMOD; ++i) {
 #pragma unroll
@@ -682,8 +707,8 @@ __global__ __launch_bounds__(NT) void crt_combine_kernel(CrtCombineArgs p) {
         cre = r0;
         cim = r1;
       }
-      frac_mac(are[e], cre, sg[i][0]);
-      if (KIND != 2) frac_mac(aim[e], cim, sg[i][1]);
+      frac_mac(are[e], (double)cre, sg[i][0]);
+      if (KIND != 2) frac_mac(aim[e], (double)cim, sg[i][1]);
     }
   }
   const double fb = p.facB[(int64_t)b * p.Np + n];
@@ -934,8 +959,9 @@ void run_crt(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca,
     pa.nout = n * planes;
     pack_planes(pa.pl, n, split, planes, cplx);
     dim3 grid((unsigned)(p.Kp / CPK), (unsigned)(Rp / CPR), (unsigned)d.B);
-    if (cplx) hipLaunchKernelGGL(crt_pack_kernel<true>, grid, dim3(NT), 0, s, pa);
-    else hipLaunchKernelGGL(crt_pack_kernel<false>, grid, dim3(NT), 0, s, pa);
+    if (split) hipLaunchKernelGGL((crt_pack_kernel<true, true>), grid, dim3(NT), 0, s, pa);
+    else if (cplx) hipLaunchKernelGGL((crt_pack_kernel<true, false>), grid, dim3(NT), 0, s, pa);
+    else hipLaunchKernelGGL((crt_pack_kernel<false, false>), grid, dim3(NT), 0, s, pa);
   };
   pack(A, TAB_ROWA, TAB_KA, TAB_BA, d.M, p.Mp, ca, !p.a_kmajor, p.planes_a, w.facA, w.shiftA, w.QA);
   pack(B, TAB_COLB, TAB_KB, TAB_BB, d.N, p.Np, cb, !p.b_kmajor, p.planes_b, w.facB, w.shiftB, w.QB);
@@ -999,9 +1025,9 @@ void run_crt(Handle& h, Plan& p, void* C, const void* A, const void* B, bool ca,
       yr = y * half % pi;
       yi = y * half % pi * mt.j[i] % pi;
     }
-    auto limbs = [pi](int64_t yv, uint32_t* out) {  // round(yv 2^96 / p) in 16-bit limbs
+    auto limbs = [pi](int64_t yv, double* out) {  // round(yv 2^96 / p) in 32-bit parts
       const unsigned __int128 G = ((((unsigned __int128)yv) << 96) + (unsigned)(pi / 2)) / (unsigned)pi;
-      for (int l = 0; l < 6; ++l) out[l] = (uint32_t)(G >> (16 * l)) & 0xffffu;
+      for (int l = 0; l < 3; ++l) out[l] = (double)(uint32_t)(G >> (32 * l));
     };
     limbs(yr, cm.gre[i]);
     limbs(yi, cm.gim[i]);
