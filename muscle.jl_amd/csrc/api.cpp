@@ -929,6 +929,8 @@ int muscle_hip_describe(const muscle_hip_tensor_t* c, const muscle_hip_tensor_t*
     check_tensor(c, "C", false);
     check_tensor(a, "A", false);
     check_tensor(b, "B", false);
+    if (precision < MUSCLE_HIP_PRECISION_NATIVE || precision > MUSCLE_HIP_PRECISION_OZAKI_INT8)
+      fail(MUSCLE_HIP_INVALID_ARGUMENT, "unknown precision mode");
     ContractionDesc d = lower_binary(*c, *a, *b);
     std::unique_ptr<Plan> p = build_contraction_plan(d, precision);
     if (!(target_rel_error > 0)) target_rel_error = 1e-12;
