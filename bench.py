@@ -350,11 +350,35 @@ def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, thre
     flops = [sum(8.0 * 2 ** len(set(s.leaves[a] if a < len(s.leaves) else s.nodes[a - len(s.leaves)][2]) |
                               set(s.leaves[b] if b < len(s.leaves) else s.nodes[b - len(s.leaves)][2]))
                  for a, b, _ in s.nodes) for s in specs]
+    # the served path's own bounds: per node, the MFMA time of the products the
+    # kernels issue (3M: 6 of the 8 algorithmic flops per complex MAC) at the
+    # dtype's dense peak (FP64 78.6, FP32 157.3 TF/s), and the larger of that
+    # and the HBM time of its operands and output (every element once, 8 TB/s)
+    def labels(s, x):
+        return set(s.leaves[x]) if x < len(s.leaves) else set(s.nodes[x - len(s.leaves)][2])
+    t_mfma, t_roof = [], []
+    for s in specs:
+        peak = (PEAK_TFLOPS["fp64"] if s.dtype == "c64" else PEAK_TFLOPS["fp32"]) * 1e12
+        es = 16 if s.dtype == "c64" else 8
+        tm = tr = 0.0
+        for a, b, o in s.nodes:
+            la, lb = labels(s, a), labels(s, b)
+            f = 6.0 * 2 ** len(la | lb) / peak
+            by = (2 ** len(la) + 2 ** len(lb) + 2 ** len(set(o))) * es / (HBM_TBS * 1e12)
+            tm += f
+            tr += max(f, by)
+        t_mfma.append(tm)
+        t_roof.append(tr)
     res = {"distribution": f"trees of 20-300 binary_einsums, dims 2, tensor sizes log-uniform in 2^{size[0]}.."
                            f"2^{size[1]}, ComplexF32/ComplexF64 per job",
            "jobs": n_jobs, "c64_jobs": sum(s.dtype == "c64" for s in specs),
            "nodes": sum(len(s.nodes) for s in specs), "tflop_total": round(sum(flops) / 1e12, 3),
-           "leaf_bank_gib": round(bank.bytes() / 2 ** 30, 2), "pool_gib": round(pool / 2 ** 30, 2)}
+           "leaf_bank_gib": round(bank.bytes() / 2 ** 30, 2), "pool_gib": round(pool / 2 ** 30, 2),
+           "ceiling_jobs_per_s_mfma": round(n_jobs / sum(t_mfma), 2),
+           "ceiling_jobs_per_s_roofline": round(n_jobs / sum(t_roof), 2),
+           "ceiling_note": "jobs/s if every node ran at its bound with no launch or scheduling gaps: MFMA "
+                           "(3M issued flops at the dtype's dense peak) or the larger of that and the node's "
+                           "HBM bytes at 8 TB/s"}
     # sample of jobs checked against the oracle: the cheapest of each dtype
     order = sorted(range(n_jobs), key=lambda i: flops[i])
     keep = [next(i for i in order if specs[i].dtype == dt) for dt in ("c64", "c32")][:check_jobs]
@@ -378,6 +402,9 @@ def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, thre
         res["batched"].append(r)
     best = max(r["jobs_per_s"] for r in res["batched"])
     res["speedup_vs_plain_loop"] = round(best / serial["jobs_per_s"], 3)  # capacity over capacity
+    res["served_frac_of_roofline_ceiling"] = round(best / res["ceiling_jobs_per_s_roofline"], 4)
+    res["served_frac_of_mfma_ceiling"] = round(best / res["ceiling_jobs_per_s_mfma"], 4)
+    res["plain_loop_frac_of_roofline_ceiling"] = round(serial["jobs_per_s"] / res["ceiling_jobs_per_s_roofline"], 4)
     if keep and threads > 0:
         from threadpoolctl import threadpool_limits
 
@@ -547,7 +574,7 @@ def main() -> int:
     ap.add_argument("--batch-budget-gib", type=float, default=96.0)
     ap.add_argument("--batch-dtypes", default="c64,c32", help="element types of the batched workload")
     ap.add_argument("--batch-loop-steps", type=int, default=1, help="timed runs of the plain per-call loop (0: none)")
-    ap.add_argument("--served-jobs", type=int, default=32, help="jobs per rate, full-size trees")
+    ap.add_argument("--served-jobs", type=int, default=200, help="jobs per rate, full-size trees")
     ap.add_argument("--served-small-jobs", type=int, default=200, help="jobs per rate, small trees")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="after the headline, copy this many GiB device-to-device 3 times (a kernel of known "
@@ -727,11 +754,13 @@ def main() -> int:
                 extra[f"batched_{bdt}"] = {"error": repr(e)}
                 log(f"batched {bdt} failed: {e!r}")
     if want("served"):
+        # offered rates straddle each distribution's capacity (spec: ~17 jobs/s, small: ~3,500 jobs/s)
+        rates = {"small": [50, 200, 1000, 2000, 0], "spec": [10, 16, 25, 0]}
         for dist, n_jobs in (("small", args.served_small_jobs), ("spec", args.served_jobs)):
             if n_jobs <= 0:
                 continue
             try:
-                extra[f"served_{dist}"] = bench_served(dist, n_jobs, [50, 200, 1000, 2000, 0], 1000, 2,
+                extra[f"served_{dist}"] = bench_served(dist, n_jobs, rates[dist], 1000, 2,
                                                        0 if args.no_cpu_baseline else threads)
             except Exception as e:  # reported in the JSON, not hidden
                 extra[f"served_{dist}"] = {"error": repr(e)}
