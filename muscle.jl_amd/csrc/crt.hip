@@ -51,6 +51,7 @@
 #include "crt.hpp"
 #include "device_types.hpp"
 #include "kernels.hpp"
+#include "lds_stage.hpp"
 
 namespace mh {
 
@@ -459,29 +460,8 @@ struct CrtGemmArgs {
   CrtConsts m;
 };
 
-constexpr int GBM = 256, GBN = 256, GKB = 128;  // tile rows, columns, K bytes per stage
-constexpr int GTILE = GBM * GKB;                 // bytes of one operand's stage in LDS (32 KiB)
-
-// Byte offset of 16-byte chunk c (0..7) of row r in an LDS operand stage
-// (128-byte rows).  The chunk index is XORed with f(r) = (r & 7) ^ ((r >> 3) & 1):
-// 8 consecutive rows at one chunk (the lanes of an MFMA fragment read that
-// share a 128-byte bank window) land in 8 distinct 16-byte bank groups, and
-// so do 16 consecutive rows within a 256-byte window (rows r and r + 8 differ
-// in f's low bit); the 8 chunks of one row are a permutation of the row.
-// (rocprofv3: no LDS bank conflicts in this kernel.)
-__device__ __forceinline__ int lds_swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
-__device__ __forceinline__ int lds_off(int r, int c) { return r * GKB + ((c ^ lds_swz(r)) << 4); }
-
-// Copies 16 bytes per lane from global memory to LDS (m0 + 16 * lane) without
-// passing through registers (global_load_lds_dwordx4).  Issued as inline
-// assembly so that the kernel places its own vmcnt wait; the instruction
-// writes no register.  The s_nop is the wait state an LDS-DMA load needs
-// after an SALU write of m0.
-__device__ __forceinline__ void lds_dma16(const void* g, uint32_t lds_addr) {
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_addr), "v"(g)
-               : "memory", "m0");
-}
-__device__ __forceinline__ void wait_vmcnt0() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+constexpr int GBM = kStageRows, GBN = kStageRows, GKB = kStageBytes;  // tile rows, columns, K bytes per stage
+constexpr int GTILE = kStageTile;                                      // one operand's stage in LDS (32 KiB)
 
 constexpr int GNT = 512;  // threads of the GEMM workgroup: 8 waves, two per SIMD
 
@@ -525,10 +505,7 @@ __global__ __launch_bounds__(GNT, 1) void crt_gemm_kernel(CrtGemmArgs p) {
   // wave, wave + 8, wave + 16, wave + 24 of 8 rows each
   int64_t goff[4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int r = (wave + 8 * t) * 8 + (lane >> 3);
-    goff[t] = (int64_t)r * p.Kp + (((lane & 7) ^ lds_swz(r)) << 4);
-  }
+  for (int t = 0; t < 4; ++t) goff[t] = stage_goff(wave, lane, t, p.Kp);
   const uint32_t lds_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t*)&lds[0][0][0];
   auto issue = [&](int st) {
     const int buf = st & 1;

@@ -59,6 +59,7 @@ struct Plan {
   int slices = 0;
   int split_mode = -1;  // 0 int8 base-255 digits (Ozaki), 1 bf16 x3, 2 fp16 x2, 3 modular (CRT)
   bool crt_split = false;  // modular form: complex products in Z_p x Z_p (two GEMMs per modulus)
+  bool split_wide = false;  // sliced form on the 256 x 256 slice GEMM (large plans)
   int planes_a = 0, planes_b = 0;
   int64_t Mp = 0, Np = 0, Kp = 0;  // padded extents of the packed operands
   int esize = 1;                   // bytes per packed element

@@ -32,6 +32,7 @@
 #include "crt.hpp"
 #include "device_types.hpp"
 #include "kernels.hpp"
+#include "lds_stage.hpp"
 #include "split.hpp"
 
 namespace mh {
@@ -43,6 +44,8 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 enum { MODE_I8 = 0, MODE_BF16 = 1, MODE_F16 = 2 };
 
@@ -441,6 +444,155 @@ __global__ __launch_bounds__(NT, 2) void sliced_gemm_kernel(SlicedArgs p) {
       }
 }
 
+// ------------------------------------------------------------------ wide slice GEMM
+// Large plans: 256 x 256 output tile per 512-thread workgroup, eight waves in
+// 2 x 4, each a 128 x 64 block of 4 x 2 MFMA 32x32 tiles (int8 32x32x32 with
+// int32 sums, or bf16/f16 32x32x16 with FP32 sums; every operand fragment is
+// 16 bytes of K per lane, lane l holding row l % 32 and K bytes
+// [16 * (l / 32), +16) of a 32-byte k-step).  128-byte K stages of A and B
+// are double-buffered in LDS (128 KiB), copied global -> LDS by the direct
+// path in the swizzled layout of lds_stage.hpp; the stages walk the group's
+// segments (each a K range of one (A slice, B slice) pair, a whole number of
+// stages long).  Next k-step's fragments are read while the current MFMAs
+// run, and the MFMAs issue at raised wave priority.
+template <int MODE>
+struct WideMma;
+template <>
+struct WideMma<MODE_I8> {
+  typedef i32x16 acc_t;
+  static __device__ inline acc_t mma(i32x4 a, i32x4 b, acc_t c) {
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct WideMma<MODE_BF16> {
+  typedef f32x16 acc_t;
+  static __device__ inline acc_t mma(i32x4 a, i32x4 b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+  }
+};
+template <>
+struct WideMma<MODE_F16> {
+  typedef f32x16 acc_t;
+  static __device__ inline acc_t mma(i32x4 a, i32x4 b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                  0, 0);
+  }
+};
+
+constexpr int WNT = 512;
+
+template <int MODE>
+__global__ __launch_bounds__(WNT, 1) void sliced_gemm_wide_kernel(SlicedArgs p) {
+  using MM = WideMma<MODE>;
+  using acc_t = typename MM::acc_t;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2][2][kStageTile];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  int tm, tn;
+  gemm_tile_coords(blockIdx.x, gridDim.x, p.tiles_m, p.tiles_n, tm, tn);
+  const int m0 = tm * kStageRows, n0 = tn * kStageRows;
+  const int b = blockIdx.y;
+  const SliceGroup grp = p.groups[blockIdx.z];
+  const uint8_t* QA = p.QA + b * p.a_b_stride + (int64_t)m0 * p.rowbytes;
+  const uint8_t* QB = p.QB + b * p.b_b_stride + (int64_t)n0 * p.rowbytes;
+
+  acc_t acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
+
+  int nstage = 0;
+  for (int s = 0; s < grp.count; ++s) {
+    const SliceSeg sg = p.segs[grp.first + s];
+    nstage += (sg.k1 - sg.k0) * p.esize / kStageBytes;
+  }
+  // the stage copied next: segment cs, K byte ckb, slot offsets of A and B
+  int cs = 0;
+  SliceSeg csg = p.segs[grp.first];
+  int ckb = csg.k0 * p.esize;
+  int64_t aseg = slot_offset(csg.a, p.S, p.a_plane_stride, p.slot_bytes);
+  int64_t bseg = slot_offset(csg.b, p.S, p.b_plane_stride, p.slot_bytes);
+  int64_t goff[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) goff[t] = stage_goff(wave, lane, t, p.rowbytes);
+  const uint32_t lds_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t*)&lds[0][0][0];
+  auto issue = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t blk = (uint32_t)(wave + 8 * t) * 1024u;
+      const uint32_t la = __builtin_amdgcn_readfirstlane(lds_base + (uint32_t)(buf * 2 + 0) * kStageTile + blk);
+      const uint32_t lb = __builtin_amdgcn_readfirstlane(lds_base + (uint32_t)(buf * 2 + 1) * kStageTile + blk);
+      lds_dma16(QA + aseg + ckb + goff[t], la);
+      lds_dma16(QB + bseg + ckb + goff[t], lb);
+    }
+    ckb += kStageBytes;
+    if (ckb >= csg.k1 * p.esize && cs + 1 < grp.count) {
+      ++cs;
+      csg = p.segs[grp.first + cs];
+      ckb = csg.k0 * p.esize;
+      aseg = slot_offset(csg.a, p.S, p.a_plane_stride, p.slot_bytes);
+      bseg = slot_offset(csg.b, p.S, p.b_plane_stride, p.slot_bytes);
+    }
+  };
+  const int fr = lane & 31, fc = lane >> 5;
+  int aoff[4], boff[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) aoff[i] = wm * 128 + i * 32 + fr;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) boff[j] = wn * 64 + j * 32 + fr;
+  i32x4 fa[2][4], fb[2][2];
+  auto read_frags = [&](int set, int buf, int ks) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      fa[set][i] = *reinterpret_cast<const i32x4*>(&lds[buf][0][lds_off(aoff[i], 2 * ks + fc)]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      fb[set][j] = *reinterpret_cast<const i32x4*>(&lds[buf][1][lds_off(boff[j], 2 * ks + fc)]);
+  };
+  issue(0);
+  for (int st = 0; st < nstage; ++st) {
+    const int buf = st & 1;
+    wait_vmcnt0();    // this wave's copies of stage st are in
+    __syncthreads();  // and every wave's; stage st - 1's buffer is free
+    if (st + 1 < nstage) issue(buf ^ 1);
+    read_frags(0, buf, 0);
+#pragma unroll
+    for (int ks = 0; ks < kStageBytes / 32; ++ks) {
+      const int cur = ks & 1;
+      if (ks + 1 < kStageBytes / 32) read_frags(cur ^ 1, buf, ks + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = MM::mma(fa[cur][i], fb[cur][j], acc[i][j]);
+      __builtin_amdgcn_s_setprio(0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // group sums: [G][B][Mp][Np]; D of a 32x32 MFMA: lane l, reg r holds row
+  // 8 * (r / 4) + 4 * (l / 32) + r % 4 and column l % 32
+  const int64_t obase = ((int64_t)blockIdx.z * p.B + b) * p.Mp;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = m0 + wm * 128 + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+        const int64_t o = (obase + m) * p.Np + n;
+        if constexpr (MODE == MODE_I8) reinterpret_cast<int32_t*>(p.out)[o] = acc[i][j][r];
+        else reinterpret_cast<float*>(p.out)[o] = acc[i][j][r];
+      }
+    }
+}
+
 // ------------------------------------------------------------------ combine
 struct CombineArgs {
   const void* G32;
@@ -631,8 +783,11 @@ void prepare_split(Plan& p, double target) {
     default: fail(MUSCLE_HIP_INVALID_ARGUMENT, "unknown precision mode");
   }
   const int64_t kstep = SKB / p.esize;  // elements per stage
-  p.Mp = round_up(d.M, SBM);
-  p.Np = round_up(d.N, SBN);
+  // large plans run on the 256 x 256 slice GEMM (rows padded to 256)
+  p.split_wide = d.M >= 256 && d.N >= 256 && d.M * d.N >= (int64_t)512 * 512 &&
+                 (double)d.M * (double)d.N * (double)d.K * (double)d.B >= 1e9;
+  p.Mp = round_up(d.M, p.split_wide ? kStageRows : SBM);
+  p.Np = round_up(d.N, p.split_wide ? kStageRows : SBN);
   p.Kp = round_up(d.K, kstep);
   kchunk = round_up(std::min<int64_t>(kchunk, p.Kp), kstep);
   const int nks = (int)((p.Kp + kchunk - 1) / kchunk);
@@ -775,15 +930,24 @@ void run_split(Handle& h, Plan& p, void* C, const void* A, const void* B, bool c
   ga.Mp = (int32_t)p.Mp;
   ga.Np = (int32_t)p.Np;
   ga.B = (int32_t)d.B;
-  ga.tiles_m = (int32_t)(p.Mp / SBM);
-  ga.tiles_n = (int32_t)(p.Np / SBN);
+  const int tile = p.split_wide ? kStageRows : SBM;
+  ga.tiles_m = (int32_t)(p.Mp / tile);
+  ga.tiles_n = (int32_t)(p.Np / tile);
   ga.esize = p.esize;
   {
     dim3 grid((unsigned)(ga.tiles_m * ga.tiles_n), (unsigned)d.B, (unsigned)p.ngroups);
-    switch (p.split_mode) {
-      case MODE_I8: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_I8>, grid, dim3(NT), 0, s, ga); break;
-      case MODE_BF16: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_BF16>, grid, dim3(NT), 0, s, ga); break;
-      default: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_F16>, grid, dim3(NT), 0, s, ga); break;
+    if (p.split_wide) {
+      switch (p.split_mode) {
+        case MODE_I8: hipLaunchKernelGGL(sliced_gemm_wide_kernel<MODE_I8>, grid, dim3(WNT), 0, s, ga); break;
+        case MODE_BF16: hipLaunchKernelGGL(sliced_gemm_wide_kernel<MODE_BF16>, grid, dim3(WNT), 0, s, ga); break;
+        default: hipLaunchKernelGGL(sliced_gemm_wide_kernel<MODE_F16>, grid, dim3(WNT), 0, s, ga); break;
+      }
+    } else {
+      switch (p.split_mode) {
+        case MODE_I8: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_I8>, grid, dim3(NT), 0, s, ga); break;
+        case MODE_BF16: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_BF16>, grid, dim3(NT), 0, s, ga); break;
+        default: hipLaunchKernelGGL(sliced_gemm_kernel<MODE_F16>, grid, dim3(NT), 0, s, ga); break;
+      }
     }
   }
   CombineArgs cm{base + w.G32,

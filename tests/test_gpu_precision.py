@@ -80,6 +80,30 @@ CRT_SHAPES = [
 ]
 
 
+# The same large shapes on the sliced rungs: plans with M, N >= 256, M*N >= 512^2
+# and M*N*K*batch >= 1e9 run their slice products on the 256 x 256 slice GEMM
+# (32x32 MFMAs).  Ozaki int8 at 3e-15 is beyond the modular form's reach
+# (2^50 integers): 7 base-255 slices on the same GEMM in int8.
+@pytest.mark.parametrize("mode", ["bf16x3", "fp16x2", "ozaki_3e-15"])
+@pytest.mark.parametrize("T", [np.complex128, np.float64])
+@pytest.mark.parametrize("shape", CRT_SHAPES, ids=lambda s: s[0])
+def test_wide_slice_rungs(shape, T, mode):
+    name, ia, sa, ib, sb, ic = shape
+    rng = np.random.default_rng(12)
+    A, B = randn(rng, sa, T), randn(rng, sb, T)
+    ref = bb.binary_einsum_general(ic, A, ia, B, ib)
+    if mode == "ozaki_3e-15":
+        C, info = run(A, ia, B, ib, ic, precision="ozaki_int8", target_rel_error=3e-15)
+        assert info["kernel"] == 3 and info["slices"] == 7, info
+        tol = 3e-15
+    else:
+        C, info = run(A, ia, B, ib, ic, precision=mode)
+        assert info["kernel"] == 3 and info["precision"] == muscle._lib.PRECISION[mode], info
+        tol = max(TOL[mode], 2 * info["err_bound"])
+    assert C.dtype == T
+    assert rel_err(C, ref) <= tol, (mode, rel_err(C, ref), tol)
+
+
 @pytest.mark.parametrize("target, kernel", CRT_TARGETS)
 @pytest.mark.parametrize("shape", CRT_SHAPES, ids=lambda s: s[0])
 def test_ozaki_int8_crt(shape, target, kernel):
