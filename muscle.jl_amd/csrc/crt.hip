@@ -3,20 +3,23 @@
 //
 //   1. norms:   per row of A and column of B the 2-norm over K (|re|^2 + |im|^2
 //               summed in FP64 with a running power-of-two scale, so no square
-//               over- or underflows); the row's scale s = 2^T / norm makes
-//               every scaled row's norm at most 2^T.
+//               over- or underflows), as partial sums per 128-wide K tile
+//               merged by one wavefront per row; the row's scale
+//               s = 2^T / norm makes every scaled row's norm at most 2^T.
 //   2. pack:    every value is gathered once, scaled and rounded to an integer
-//               (exact in FP64: |x| <= 2^T < 2^53), and written as its
+//               (exact in FP64: |x| <= 2^T <= 2^50) and written as its
 //               symmetric residues modulo each of N pairwise coprime moduli
-//               p_i <= 256, one int8 plane per modulus and operand plane.
+//               p_i <= 256, one int8 plane per modulus and operand plane; the
+//               residues come from byte dot products of x's base-256 digits.
 //   3. GEMM:    for every modulus and product, an int8 MFMA GEMM with exact
 //               int32 accumulation (K * 128^2 < 2^31); the epilogue keeps the
 //               result modulo p_i as one int8 per output.
 //   4. combine: per output, the residues c_i of re and im give
-//               x / M = sum_i ((c_i y_i) mod p_i) / p_i (mod 1), with
-//               y_i = (M / p_i)^-1 mod p_i; x = M * (that fraction, centred) is
-//               exact because |x| <= ||A_row|| ||B_col|| <= 2^(2T) < M / 2
-//               (Cauchy-Schwarz), and C = x / (s_row s_col) in FP64.
+//               x / M = sum_i c_i y_i / p_i (mod 1), with
+//               y_i = (M / p_i)^-1 mod p_i, summed exactly in 96-bit fixed
+//               point; x = M * (that fraction, centred) is exact because
+//               |x| <= ||A_row|| ||B_col|| <= 2^(2T) < M / 2 (Cauchy-Schwarz),
+//               and C = x / (s_row s_col) in FP64.
 //
 // Complex x complex products take one of two forms:
 //   split:  every modulus p has a square root j of -1 (all prime factors of p
