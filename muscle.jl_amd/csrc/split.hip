@@ -629,26 +629,26 @@ __global__ __launch_bounds__(NT) void combine_kernel(CombineArgs p) {
   const int64_t plane = (int64_t)p.B * p.Mp * p.Np;
   const int64_t at = ((int64_t)b * p.Mp + m) * p.Np + n0;
   double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int g = 0; g < p.G; ++g) {
-    double v[4];
-    if (p.is_int) {
-      const int4 x = *reinterpret_cast<const int4*>(reinterpret_cast<const int32_t*>(p.G32) + g * plane + at);
-      v[0] = x.x;
-      v[1] = x.y;
-      v[2] = x.z;
-      v[3] = x.w;
-    } else {
-      const float4 x = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(p.G32) + g * plane + at);
-      v[0] = x.x;
-      v[1] = x.y;
-      v[2] = x.z;
-      v[3] = x.w;
-    }
-    const double cr = p.coef[g], ci = p.coef[p.G + g];
+  // groups in blocks of 8: the 8 loads are in flight together (small plans
+  // have few outputs and up to hundreds of groups), the sums stay in group order
+  constexpr int GU = 8;
+  const uint32_t* G32 = reinterpret_cast<const uint32_t*>(p.G32) + at;
+  for (int g0 = 0; g0 < p.G; g0 += GU) {
+    uint4 x[GU];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      re[j] = fma(cr, v[j], re[j]);
-      im[j] = fma(ci, v[j], im[j]);
+    for (int u = 0; u < GU; ++u)
+      x[u] = g0 + u < p.G ? *reinterpret_cast<const uint4*>(G32 + (int64_t)(g0 + u) * plane) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      if (g0 + u >= p.G) break;
+      const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+      const double cr = p.coef[g0 + u], ci = p.coef[p.G + g0 + u];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double v = p.is_int ? (double)(int32_t)w[j] : (double)__uint_as_float(w[j]);
+        re[j] = fma(cr, v, re[j]);
+        im[j] = fma(ci, v, im[j]);
+      }
     }
   }
   const int ea = p.expA ? p.expA[(int64_t)b * p.M + m] : 0;
@@ -790,7 +790,6 @@ void prepare_split(Plan& p, double target) {
   p.Np = round_up(d.N, p.split_wide ? kStageRows : SBN);
   p.Kp = round_up(d.K, kstep);
   kchunk = round_up(std::min<int64_t>(kchunk, p.Kp), kstep);
-  const int nks = (int)((p.Kp + kchunk - 1) / kchunk);
 
   // slice pairs grouped by weight
   std::vector<std::vector<std::pair<int, int>>> pair_groups;  // 1-based slice indices
@@ -811,6 +810,20 @@ void prepare_split(Plan& p, double target) {
     weights.push_back(1.0);
   }
   const std::vector<Product> prods = products_for(ca, cb);
+  // small plans (few output tiles): shorter K chunks, down to one stage, so
+  // that tiles x groups fills the chip (every chunk is one more group; FP32
+  // sums over shorter chunks are only more accurate, int8 sums stay exact)
+  {
+    const int64_t tile = p.split_wide ? kStageRows : SBM;
+    const int64_t tiles = (p.Mp / tile) * (p.Np / tile) * d.B;
+    const int64_t per_chunk = tiles * (int64_t)prods.size() * (int64_t)pair_groups.size();
+    const int64_t want_chunks = (512 + per_chunk - 1) / per_chunk;
+    if (want_chunks > 1) {
+      const int64_t kc = round_up((p.Kp + want_chunks - 1) / want_chunks, kstep);
+      kchunk = std::max<int64_t>(kstep, std::min<int64_t>(kchunk, kc));
+    }
+  }
+  const int nks = (int)((p.Kp + kchunk - 1) / kchunk);
   p.split_segs.clear();
   p.split_groups.clear();
   std::vector<double> cre, cim;
