@@ -1,5 +1,6 @@
-// Batched contraction kernels: the persistent MFMA kernel (pairs with K > 16)
-// and the shallow vector-FMA kernel (pairs with K <= 16, further below).
+// Batched contraction kernels: the persistent MFMA kernel (pairs with K > 16),
+// the shallow vector-FMA kernel (pairs with K <= 16) and the skinny vector-FMA
+// kernel (pairs with an output side of at most 16), the last two further below.
 //
 // Thousands of uneven pairwise contractions (the pairs of a contraction
 // tree) run in one launch: a fixed grid of workgroups (2 per CU) pulls
@@ -446,6 +447,7 @@ __device__ __forceinline__ void shallow_tile(const BatchPair& pr, int BM, int BN
 }
 
 // 6 waves per SIMD: three 8-wave workgroups per CU (80 registers, no spills)
+// 6 waves per SIMD: three 8-wave workgroups per CU (80 registers, no spills)
 template <typename T, bool CP>
 __global__ __launch_bounds__(SNT) __attribute__((amdgpu_waves_per_eu(6, 8))) void shallow_kernel(
     ShallowArgs a) {
@@ -484,6 +486,158 @@ __global__ __launch_bounds__(SNT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     __syncthreads();
     if (pr.c_mfast) shallow_tile<T, CP, true>(pr, BM, BN, mr, nr, sab, soff);
     else shallow_tile<T, CP, false>(pr, BM, BN, mr, nr, sab, soff);
+  }
+}
+
+// ---- skinny pairs ---------------------------------------------------------
+// A pair whose output has a short side of S <= kSkinnyMaxS elements does at
+// most S complex products per element of the long side's operand it reads,
+// so it is bound by streaming that operand and C, and an MFMA or shallow tile
+// (at least 32 wide) would spend most of its work on padding.  Here one
+// workgroup of 512 threads covers the whole short side (padded to a power of
+// two) and lt positions of the long side (skinny_geom):
+//   - each thread owns J long positions and Q = O / J short positions (O = 4
+//     outputs for complex FP64, else 8); it reads its long positions' operand
+//     straight from memory into registers, once per (position, k), from a
+//     ring of O loads refilled D = O / J steps of k ahead;
+//   - the short side's operand is staged in LDS in chunks of KSB elements
+//     (row stride sp) and read as wave-wide broadcasts;
+//   - the item's K offsets (K <= kSkinnyMaxK) and short-side offsets are
+//     looked up once, into LDS;
+//   - C is stored with consecutive lanes on consecutive long positions.
+// Each output sums its K products in increasing k with the shallow kernel's
+// complex product (a from A, b from B), so a pair's result does not depend on
+// the orientation or on what else is in the batch.
+constexpr int KNT = 512;   // threads per workgroup
+constexpr int KSB = 2048;  // short-side operand elements staged per chunk
+constexpr int KOFF_S = 2 * kSkinnyMaxK, KOFF_C = KOFF_S + kSkinnyMaxS, KOFF_N = KOFF_C + kSkinnyMaxS;
+
+// SM: the short side is M (A is the short operand, B the long one, C's long
+// direction is N); else the short side is N.
+template <typename T, bool CP, int J, bool SM>
+__device__ __forceinline__ void skinny_tile(const BatchPair& pr, int bz, int tl, Cplx<T>* sS, int64_t* sK) {
+  constexpr int O = (CP && sizeof(T) == 8) ? 4 : 8;
+  constexpr int QM = O / J, D = O / J;
+  const int tid = threadIdx.x;
+  const int S = SM ? pr.M : pr.N, L = SM ? pr.N : pr.M, K = pr.K;
+  const SkinnyGeom g = skinny_geom(S, O);
+  const int sps = __builtin_ctz(g.sp);
+  const int kc = KSB / g.sp;  // K per staged chunk, a multiple of D
+  const int q0 = (tid / g.lw) * QM;
+  const int lbase = tl * g.lt + tid % g.lw;
+  const void* XL = SM ? pr.B : pr.A;
+  const void* XS = SM ? pr.A : pr.B;
+  const OffTab& tabL = SM ? pr.colB : pr.rowA;
+  const OffTab& tabS = SM ? pr.rowA : pr.colB;
+  const OffTab& kL = SM ? pr.kB : pr.kA;
+  const OffTab& kS = SM ? pr.kA : pr.kB;
+  const OffTab& cL = SM ? pr.colC : pr.rowC;
+  const OffTab& cS = SM ? pr.rowC : pr.colC;
+  const int64_t bS = SM ? pr.bA[bz] : pr.bB[bz];
+  const int64_t bL = SM ? pr.bB[bz] : pr.bA[bz];
+
+  __syncthreads();  // the previous item's readers of LDS are done
+  for (int k = tid; k < K; k += KNT) {
+    sK[k] = kL[k];
+    sK[kSkinnyMaxK + k] = kS[k];
+  }
+  if (tid < kSkinnyMaxS) {
+    const bool ok = tid < S;
+    sK[KOFF_S + tid] = ok ? bS + tabS[tid] : 0;
+    sK[KOFF_C + tid] = ok ? cS[tid] : 0;
+  }
+  // this thread's long positions (clamped into range; their outputs are not stored)
+  int64_t oL[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int l = lbase + g.lw * j;
+    oL[j] = bL + tabL[l < L ? l : L - 1];
+  }
+  __syncthreads();
+
+  // the short operand's chunk of K starting at kb, as sS[kk * sp + q]
+  auto stage = [&](int kb) {
+    const int kn = min(kc, K - kb);
+    Cplx<T> r[KSB / KNT];
+#pragma unroll
+    for (int i = 0; i < KSB / KNT; ++i) {
+      const int e = tid + KNT * i;
+      const int kk = e >> sps, q = e & (g.sp - 1);
+      r[i] = Cplx<T>{T(0), T(0)};
+      if (kk < kn && q < S) r[i] = load_elem<T, CP>(XS, sK[KOFF_S + q] + sK[kSkinnyMaxK + kb + kk]);
+    }
+#pragma unroll
+    for (int i = 0; i < KSB / KNT; ++i) sS[tid + KNT * i] = r[i];
+  };
+
+  Cplx<T> acc[J][QM];
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int i = 0; i < QM; ++i) acc[j][i] = Cplx<T>{T(0), T(0)};
+  Cplx<T> ring[D][J];
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+#pragma unroll
+    for (int j = 0; j < J; ++j) ring[d][j] = d < K ? load_elem<T, CP>(XL, oL[j] + sK[d]) : Cplx<T>{T(0), T(0)};
+  if (K > 0) stage(0);
+  __syncthreads();
+  int kb = 0;
+  for (int k0 = 0; k0 < K; k0 += D) {
+    if (k0 - kb >= kc) {  // the next chunk (k0 = kb + kc: kc is a multiple of D)
+      __syncthreads();
+      kb = k0;
+      stage(kb);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 + d;
+      if (k < K) {
+        const Cplx<T>* srow = sS + ((k - kb) << sps) + q0;
+#pragma unroll
+        for (int i = 0; i < QM; ++i) {
+          const Cplx<T> sv = srow[i];
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+            if constexpr (SM) cmac<T, CP>(acc[j][i], sv, ring[d][j]);
+            else cmac<T, CP>(acc[j][i], ring[d][j], sv);
+          }
+        }
+        if (k + D < K) {
+#pragma unroll
+          for (int j = 0; j < J; ++j) ring[d][j] = load_elem<T, CP>(XL, oL[j] + sK[k + D]);
+        }
+      }
+    }
+  }
+  const int64_t offC = pr.bC[bz];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int l = lbase + g.lw * j;
+    if (l >= L) continue;
+    const int64_t cl = offC + cL[l];
+#pragma unroll
+    for (int i = 0; i < QM; ++i)
+      if (q0 + i < S) store_elem<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
+  }
+}
+
+// One launch per J (the items of pairs with the same long positions per
+// thread), so each has the registers its own ring and accumulators need.
+template <typename T, bool CP, int J>
+__global__ __launch_bounds__(KNT) __attribute__((amdgpu_waves_per_eu(6, 8))) void skinny_kernel(ShallowArgs a) {
+  __shared__ Cplx<T> sS[KSB];
+  __shared__ int64_t sK[KOFF_N];
+  for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const int2 w = a.items[it];
+    const BatchPair& pr = a.pairs[w.x];
+    const bool sm = pr.M <= pr.N;
+    const SkinnyGeom g = skinny_geom(sm ? pr.M : pr.N, (CP && sizeof(T) == 8) ? 4 : 8);
+    const int tiles = ((sm ? pr.N : pr.M) + g.lt - 1) / g.lt;
+    const int bz = w.y / tiles, tl = w.y - bz * tiles;
+    if (sm) skinny_tile<T, CP, J, true>(pr, bz, tl, sS, sK);
+    else skinny_tile<T, CP, J, false>(pr, bz, tl, sS, sK);
   }
 }
 
@@ -533,6 +687,29 @@ void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s) {
     if (cplx) hipLaunchKernelGGL((shallow_kernel<float, true>), dim3(grid), dim3(SNT), 0, s, a);
     else hipLaunchKernelGGL((shallow_kernel<float, false>), dim3(grid), dim3(SNT), 0, s, a);
   }
+}
+
+void launch_skinny(bool dbl, bool cplx, int j, const ShallowArgs& a, hipStream_t s) {
+  if (a.nitems <= 0) return;
+  const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, int64_t(1) << 20);
+#define MH_SKINNY(T, CP)                                                                   \
+  switch (j) {                                                                             \
+    case 1: hipLaunchKernelGGL((skinny_kernel<T, CP, 1>), dim3(grid), dim3(KNT), 0, s, a); break; \
+    case 2: hipLaunchKernelGGL((skinny_kernel<T, CP, 2>), dim3(grid), dim3(KNT), 0, s, a); break; \
+    default: hipLaunchKernelGGL((skinny_kernel<T, CP, 4>), dim3(grid), dim3(KNT), 0, s, a); break; \
+  }
+  if (dbl) {
+    if (cplx) {
+      // complex FP64 runs 4 outputs per thread: j is 1 or 2
+      if (j == 1) hipLaunchKernelGGL((skinny_kernel<double, true, 1>), dim3(grid), dim3(KNT), 0, s, a);
+      else hipLaunchKernelGGL((skinny_kernel<double, true, 2>), dim3(grid), dim3(KNT), 0, s, a);
+    } else {
+      MH_SKINNY(double, false)
+    }
+  } else {
+    if (cplx) MH_SKINNY(float, true) else MH_SKINNY(float, false)
+  }
+#undef MH_SKINNY
 }
 
 void launch_batch(bool dbl, bool cplx, const BatchArgs& a, const BatchArgs& sp, int wg, size_t lds, hipStream_t s) {

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -51,14 +52,50 @@ struct BatchArgs {
 // instead of the persistent MFMA launch.
 constexpr int kShallowK = 16;
 
+// Pairs whose output has a side of at most kSkinnyMaxS elements and whose
+// contracted extent is at most kSkinnyMaxK run in the skinny kernel (vector
+// FMAs; one workgroup covers the whole short side and a run of the long side)
+// instead of the MFMA or shallow launches, whose tiles are at least 32 wide.
+constexpr int kSkinnyMaxS = 16;
+constexpr int kSkinnyMaxK = 1024;
+
+inline bool skinny_pair(int64_t m, int64_t n, int64_t k) {
+  return std::min(m, n) <= kSkinnyMaxS && k <= kSkinnyMaxK;
+}
+
+// Skinny item geometry for a short side of s elements and o outputs per
+// thread (4 for complex FP64, else 8): st = s padded to a power of two, j =
+// long positions per thread (at most o / 2, so that a thread's ring of o
+// operand loads runs two or more steps of k ahead), q = short positions per
+// thread (o / j), lw = threads along the long side (the 512 threads form
+// 512 / lw groups of q short positions; sp = the short side they cover),
+// lt = long positions per item (lw * j).
+struct SkinnyGeom {
+  int st, j, q, sp, lw, lt;
+};
+__host__ __device__ inline SkinnyGeom skinny_geom(int64_t s, int o) {
+  SkinnyGeom g;
+  g.st = s <= 1 ? 1 : s <= 2 ? 2 : s <= 4 ? 4 : s <= 8 ? 8 : 16;
+  g.j = o / g.st < 1 ? 1 : o / g.st > o / 2 ? o / 2 : o / g.st;
+  g.q = o / g.j;
+  g.sp = g.st > g.q ? g.st : g.q;
+  g.lw = 512 / (g.sp / g.q);
+  g.lt = g.lw * g.j;
+  return g;
+}
+__host__ __device__ inline int skinny_outputs(bool dbl, bool cplx) { return dbl && cplx ? 4 : 8; }
+
 struct ShallowArgs {
   const BatchPair* pairs;
-  const int2* items;  // the shallow pairs' tiles
+  const int2* items;  // the shallow (or skinny) pairs' items
   int64_t nitems;
 };
 
 // Launches the shallow kernel over a.nitems tiles (nothing when there are none).
 void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s);
+// Launches the skinny kernel for pairs of j long positions per thread over
+// a.nitems items (nothing when there are none).
+void launch_skinny(bool dbl, bool cplx, int j, const ShallowArgs& a, hipStream_t s);
 // Launches the persistent kernel (grid = workgroups) over a's items, then the
 // split kernel over split's items (the K slices of split pairs), preceded by a
 // kernel that resets the work counter and the split tiles' arrival counters
@@ -87,9 +124,11 @@ struct Batch {
   std::vector<std::vector<int64_t>> ext, str;
   std::vector<BatchPair> hpairs;
   // persistent items (largest K first), then the split pairs' slice items,
-  // then the shallow items
+  // then the shallow items, then the skinny items
   std::vector<int2> hitems;
-  int64_t nshallow = 0;  // the shallow items at the end of hitems
+  int64_t nskinny = 0;   // the skinny items at the end of hitems
+  int64_t nskinny_j[3] = {0, 0, 0};  // of them, those with j = 1, 2, 4 (in that order)
+  int64_t nshallow = 0;  // the shallow items before them
   int64_t nsplit = 0;    // the slice items before them
   // split K (per pair of the work list, by position): slices, K per slice,
   // offsets of the pair's partial tiles (bytes) and arrival counters

@@ -79,6 +79,26 @@ int batch_tuning_class(int64_t m, int64_t n) { return tuning_class(m, n); }
 
 void plan_pair(const Handle& h, PlannedPair& L, std::vector<int64_t>& htab, int tile, int64_t single_max) {
   const int T3[9][2] = {{0, 0}, {2, 0}, {2, 1}, {1, 1}, {0, 2}, {1, 2}, {3, 0}, {3, 1}, {3, 2}};
+  // Order of the free modes inside M and N: a tile (or skinny item) covers the
+  // lowest positions of both, so the modes along which a 128-byte line of the
+  // operand or of C runs go first, in C's stride order, then the others in C's
+  // stride order.  Each line of A, B and C is then read or written by one tile
+  // instead of being fetched again by tiles on other XCDs.  The results do not
+  // depend on this order (each output sums its K products in K's order).
+  {
+    const int64_t line = std::max<int64_t>(1, 128 / (int64_t)dtype_size(L.d.dt[2]));
+    auto absl = [](int64_t x) { return x < 0 ? -x : x; };
+    auto order = [&](std::vector<Dim>& v, int op) {
+      std::stable_sort(v.begin(), v.end(), [&](const Dim& x, const Dim& y) {
+        const bool px = absl(x.s[op]) < line || absl(x.s[2]) < line;
+        const bool py = absl(y.s[op]) < line || absl(y.s[2]) < line;
+        if (px != py) return px;
+        return absl(x.s[2]) < absl(y.s[2]);
+      });
+    };
+    order(L.d.m, 0);
+    order(L.d.n, 1);
+  }
   const ContractionDesc& d = L.d;
   const int64_t lim = int64_t(1) << 31;
   if (d.M >= lim || d.N >= lim || d.K >= lim || d.B >= lim)
@@ -145,16 +165,21 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     q.k = L.d.K;
     q.batch = L.d.B;
   }
-  std::vector<int2> shallow, sliced;
+  std::vector<int2> shallow, sliced, skinny[3];
+  const int sko = skinny_outputs(is_double(B.dtype), cp);
   for (int64_t i : order) {
     const PlannedPair& L = *lw[(size_t)i];
     if (L.d.empty) continue;
     B.flops += (cp ? 8.0 : 2.0) * (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B;
     if (L.solo) continue;
     const TileShape& ts = kTileShapes[L.tile];
-    const int64_t items = tiles_of(L);
+    // a pair with an output side of at most kSkinnyMaxS: items of the whole
+    // short side by skinny_geom's run of the long side, per batch index
+    const bool sk = skinny_pair(L.d.M, L.d.N, L.d.K);
+    const SkinnyGeom sg = skinny_geom(std::min(L.d.M, L.d.N), sko);
+    const int64_t items = sk ? (std::max(L.d.M, L.d.N) + sg.lt - 1) / sg.lt * L.d.B : tiles_of(L);
     if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
-    const bool sh = L.d.K <= kShallowK;
+    const bool sh = !sk && L.d.K <= kShallowK;
     // Split K for a pair of few tiles (at most kSplitMaxTiles) and a long K
     // (at least kSplitMinKTiles K-tiles): slices of at least 8 K-tiles, at
     // most 16 of them.  Such tiles would otherwise set a small launch's
@@ -162,7 +187,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     // depend on what else is in the launch.
     int64_t ks = 1, kc = L.d.K;
     const int64_t kt = (L.d.K + 15) / 16;
-    if (B.allow_split && !sh && kt >= kSplitMinKTiles && items <= kSplitMaxTiles) {
+    if (B.allow_split && !sh && !sk && kt >= kSplitMinKTiles && items <= kSplitMaxTiles) {
       ks = std::min<int64_t>(16, kt / 8);
       kc = (kt + ks - 1) / ks * 16;
       ks = (L.d.K + kc - 1) / kc;
@@ -177,20 +202,29 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     B.ksplit[(size_t)i] = (int32_t)ks;
     B.kchunk[(size_t)i] = (int32_t)kc;
     B.pinfo[(size_t)i].ksplit = (int32_t)ks;
-    if (sh) {
+    if (sk) {
+      B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SKINNY;
+      B.pinfo[(size_t)i].tile = -1;
+    } else if (sh) {
       B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SHALLOW;
       B.pinfo[(size_t)i].tile = -1;
     } else if (ks > 1) {
       B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SPLIT_K;
     }
-    std::vector<int2>& dst = sh ? shallow : ks > 1 ? sliced : B.hitems;
+    std::vector<int2>& dst = sk ? skinny[sg.j == 1 ? 0 : sg.j == 2 ? 1 : 2] : sh ? shallow : ks > 1 ? sliced : B.hitems;
     for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
-    if (!sh) used[L.tile] = true;
+    if (!sh && !sk) used[L.tile] = true;
   }
   B.nsplit = (int64_t)sliced.size();
   B.nshallow = (int64_t)shallow.size();
   B.hitems.insert(B.hitems.end(), sliced.begin(), sliced.end());
   B.hitems.insert(B.hitems.end(), shallow.begin(), shallow.end());
+  B.nskinny = 0;
+  for (int j = 0; j < 3; ++j) {
+    B.nskinny_j[j] = (int64_t)skinny[j].size();
+    B.nskinny += B.nskinny_j[j];
+    B.hitems.insert(B.hitems.end(), skinny[j].begin(), skinny[j].end());
+  }
   const bool dbl = is_double(B.dtype);
   for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
     if (used[t]) B.lds = std::max(B.lds, batch_lds_bytes(dbl, cp, t));
@@ -349,7 +383,7 @@ std::unique_ptr<Batch> build_batch(Handle& h, int64_t n, const muscle_hip_tensor
 }
 
 void batch_launch(Batch& b, hipStream_t s) {
-  const int64_t nmain = (int64_t)b.hitems.size() - b.nsplit - b.nshallow;
+  const int64_t nmain = (int64_t)b.hitems.size() - b.nsplit - b.nshallow - b.nskinny;
   if (nmain > 0 || b.nsplit > 0) {
     BatchArgs a;
     a.pairs = b.dpairs;
@@ -372,6 +406,18 @@ void batch_launch(Batch& b, hipStream_t s) {
     a.items = b.ditems + nmain + b.nsplit;
     a.nitems = b.nshallow;
     launch_shallow(is_double(b.dtype), is_complex(b.dtype), a, s);
+  }
+  // the skinny items, one launch per long positions per thread (1, 2, 4)
+  int64_t off = nmain + b.nsplit + b.nshallow;
+  for (int j = 0; j < 3; ++j) {
+    if (b.nskinny_j[j] > 0) {
+      ShallowArgs a;
+      a.pairs = b.dpairs;
+      a.items = b.ditems + off;
+      a.nitems = b.nskinny_j[j];
+        launch_skinny(is_double(b.dtype), is_complex(b.dtype), 1 << j, a, s);
+    }
+    off += b.nskinny_j[j];
   }
 }
 
@@ -451,6 +497,7 @@ void batch_autotune(Batch& b, int samples, int reps) {
   for (size_t i = 0; i < b.hpairs.size(); ++i) {
     const BatchPair& P = b.hpairs[i];
     if (P.M * (int64_t)P.N * P.NB == 0 || (i < b.solo.size() && b.solo[i])) continue;
+    if (skinny_pair(P.M, P.N, P.K)) continue;  // no tile shape to choose
     by_class[tuning_class(P.M, P.N)].push_back((int64_t)i);
   }
   hipEvent_t e0, e1;
@@ -547,6 +594,7 @@ int muscle_hip_batch_get_info(muscle_hip_batch_t batch, muscle_hip_batch_info_t*
     info->pairs = (int64_t)b->cs.size();
     info->items = (int64_t)b->hitems.size();
     info->shallow_items = b->nshallow;
+    info->skinny_items = b->nskinny;
     info->split_items = b->split_items;
     info->workgroups = b->workgroups;
     info->graph = b->exec ? 1 : 0;

@@ -53,6 +53,7 @@ class BatchInfo(ctypes.Structure):
         ("table_bytes", ctypes.c_int64),
         ("shallow_items", ctypes.c_int64),
         ("split_items", ctypes.c_int64),
+        ("skinny_items", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -76,7 +77,7 @@ class BatchPairInfo(ctypes.Structure):
 
 
 # batched pair routes (muscle_hip_pair_route_t)
-PAIR_ROUTES = {0: "empty", 1: "tiles", 2: "split_k", 3: "shallow", 4: "solo"}
+PAIR_ROUTES = {0: "empty", 1: "tiles", 2: "split_k", 3: "shallow", 4: "solo", 5: "skinny"}
 
 
 class JobDesc(ctypes.Structure):

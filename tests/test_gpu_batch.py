@@ -55,7 +55,8 @@ def test_batch_matches_oracle(T, tol):
     pairs, refs = make_pairs(rng, T)
     batch = ContractionBatch(pairs)
     info = batch.info()
-    assert info["pairs"] == len(pairs) and info["items"] > info["shallow_items"] > 0  # both kernels run
+    assert info["pairs"] == len(pairs) and info["items"] > info["shallow_items"] + info["skinny_items"]
+    assert info["shallow_items"] > 0 and info["skinny_items"] > 0  # all three kernels run
     batch.run("persistent")
     check(pairs, refs, tol)
     assert batch.info()["graph"] == 1
@@ -137,8 +138,12 @@ def test_pair_routes_cover_the_launches():
     info = batch.info()
     routes = [batch.pair_info(i) for i in range(len(pairs))]
     assert sum(r["route"] == "shallow" for r in routes) > 0
+    assert sum(r["route"] == "skinny" for r in routes) > 0
     for r in routes:
-        assert r["route"] in ("empty", "tiles", "split_k", "shallow", "solo")
+        if r["route"] == "skinny":
+            assert min(r["m"], r["n"]) <= 16 and r["k"] <= 1024
+    for r in routes:
+        assert r["route"] in ("empty", "tiles", "split_k", "shallow", "solo", "skinny")
         assert (r["tile"] >= 0) == (r["route"] in ("tiles", "split_k"))
         assert (r["ksplit"] > 1) == (r["route"] == "split_k")
         assert r["size_class"] == tuning_class(r["m"], r["n"])
@@ -281,7 +286,7 @@ def test_split_k_pairs(T, tol, tile):
         refs.append(bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib))
     batch = ContractionBatch(pairs, tiles=[tile] * len(pairs))
     info = batch.info()
-    assert info["split_items"] > 0 and info["shallow_items"] > 0
+    assert info["split_items"] > 0 and info["skinny_items"] > 0
     batch.run("persistent_nograph")
     check(pairs, refs, tol)
     eager = [c.parent().clone() for c, a, b in pairs]
@@ -319,3 +324,64 @@ def test_batch_contracted_extent_zero():
     assert np.array_equal(host(pairs[0][0]), np.zeros((5, 6)))
     assert rel_err(host(pairs[1][0]), A @ B) <= 1e-12
     batch.close()
+
+
+SKINNY = [
+    (["i", "j"], (1, 37), ["j", "k"], (37, 900), ["i", "k"]),  # short side 1, one row of C
+    (["i", "j"], (700, 5), ["j", "k"], (5, 3), ["i", "k"]),  # short side 3 along C's contiguous direction
+    (["j", "i"], (300, 16), ["k", "j"], (2500, 300), ["i", "k"]),  # short side 16, K = 300: three staged chunks
+    (["i", "j", "h"], (6, 40, 3), ["j", "k", "h"], (40, 1100, 3), ["h", "k", "i"]),  # hyper modes
+    (["i", "j"], (2, 1024), ["j", "k"], (1024, 2000), ["k", "i"]),  # the largest skinny K
+    (["i", "j"], (9, 1025), ["j", "k"], (1025, 40), ["i", "k"]),  # K = 1025: the MFMA route (K slices)
+    (["i"], (4000,), ["k"], (4,), ["k", "i"]),  # outer product (K = 1)
+    (["a", "b"], (5, 7), ["b", "c"], (7, 5), ["a", "c"]),  # both sides short
+    (["i", "j", "s"], (12, 30, 3), ["j", "k"], (30, 513), ["i", "k"]),  # index summed in A alone
+    (["i", "j", "i"], (10, 30, 10), ["j", "k"], (30, 700), ["i", "k"]),  # diagonal of A
+    (["i", "j"], (15, 9), ["j", "k"], (9, 5000), ["i", "k"]),  # ragged last run of the long side
+]
+
+
+@pytest.mark.parametrize("T, tol", [(np.complex128, 1e-12), (np.float64, 1e-12), (np.complex64, 2e-5),
+                                    (np.float32, 2e-5)])
+def test_skinny_pairs(T, tol):
+    """Pairs with an output side of at most 16 and K <= 1024 run in the skinny
+    vector-FMA kernel (whole short side per workgroup, long-side operand
+    streamed into registers): short sides 1-16 on either side of C, K from 1 to
+    1024 over several staged chunks, hyper, summed and diagonal modes match the
+    oracle; graph replays equal the eager launch bit for bit, and a pair's
+    result is the same alone as in the batch."""
+    rng = np.random.default_rng(21)
+    pairs, refs = [], []
+    for ia, sa, ib, sb, ic in SKINNY:
+        A, B = randn(rng, sa, T), randn(rng, sb, T)
+        a, b = dev_tensor(A, ia), dev_tensor(B, ib)
+        size = dict(zip(ia, sa)) | dict(zip(ib, sb))
+        c = muscle.Tensor(muscle.empty_col_major([size[x] for x in ic], a.dtype, a.device), ic)
+        pairs.append((c, a, b))
+        refs.append(bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib))
+    batch = ContractionBatch(pairs)
+    routes = [batch.pair_info(i)["route"] for i in range(len(pairs))]
+    assert routes == ["skinny"] * 5 + ["split_k"] + ["skinny"] * 5, routes
+    assert batch.info()["skinny_items"] > 0
+    batch.run("persistent_nograph")
+    check(pairs, refs, tol)
+    eager = [c.parent().clone() for c, a, b in pairs]
+    for rep in range(2):
+        for c, a, b in pairs:
+            c.parent().fill_(float("nan"))
+        _ = torch.randn(1000, device="cuda").sum().item()  # unrelated eager work between replays
+        batch.run("persistent")
+        torch.cuda.synchronize()
+        for (c, a, b), e in zip(pairs, eager):
+            assert torch.equal(c.parent(), e), rep
+    batch.run("loop")
+    check(pairs, refs, tol)
+    batch.close()
+    for k in (0, 2, 4):  # alone, the same bits as in the batch
+        c, a, b = pairs[k]
+        alone = ContractionBatch([(c, a, b)])
+        c.parent().fill_(float("nan"))
+        alone.run()
+        torch.cuda.synchronize()
+        assert torch.equal(c.parent(), eager[k]), k
+        alone.close()

@@ -7,7 +7,7 @@
   config 4: the 10,000-pair dims-2 workload exactly as the benchmark builds it
       (muscle.workloads.circuit_batch_workload, autotuned tile shapes), both
       element types.  A seeded sample of pairs that covers every route
-      (persistent tiles, K slices, shallow, solo) and every size class; each
+      (persistent tiles, K slices, shallow, skinny, solo) and every size class; each
       GPU output is checked against the oracle run on the GPU's own inputs.
   config 5: served jobs at the specified sizes (trees of 20-300 contractions,
       every tensor log-uniform in 2^8..2^28 elements), two per element type,
@@ -84,7 +84,7 @@ def test_config4_full_workload(dtype):
     batches = [ContractionBatch(ts) for ts in wl.rounds]
     pick, groups = _sample(batches, wl, seed=7)
     routes = {k[0] for k in groups}
-    assert {"tiles", "split_k", "shallow", "solo"} <= routes, routes
+    assert {"tiles", "split_k", "shallow", "skinny", "solo"} <= routes, routes
     print(f"config4 {dtype}: {len(wl.rounds)} rounds, {len(pick)} sampled pairs, "
           f"{len(groups)} (route, class) groups", flush=True)
     got = {}
