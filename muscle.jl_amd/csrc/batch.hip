@@ -380,112 +380,182 @@ __device__ inline void cmac(Cplx<T>& acc, const Cplx<T>& a, const Cplx<T>& b) {
   }
 }
 
-template <typename T, bool CP, bool CM>
-__device__ __forceinline__ void shallow_tile(const BatchPair& pr, int BM, int BN, int mr, int nr, Cplx<T>* sab,
-                                             const int64_t* soff) {
+// The scalars of one shallow item that outlive its table phase.
+struct ShallowItem {
+  const void* A;
+  const void* B;
+  void* C;
+  int BM, BN, mr, nr, K, a_kmajor, b_kmajor, c_mfast;
+};
+
+// Table phase of item `it`: one lookup per thread (rows, columns, K offsets,
+// batch offsets) into soff.  The caller synchronizes before soff is read.
+__device__ __forceinline__ ShallowItem shallow_tables(const ShallowArgs& a, int64_t it, int64_t* soff) {
   const int tid = threadIdx.x;
-  Cplx<T>* sA = sab;             // [k][m], BM per k
-  Cplx<T>* sB = sab + BM * SKC;  // [k][n], BN per k
+  const int2 w = a.items[it];
+  const BatchPair& pr = a.pairs[w.x];
+  ShallowItem s;
+  s.BM = pr.tile == MUSCLE_HIP_TILE_32x128 ? 32 : pr.tile == MUSCLE_HIP_TILE_128x32 ? 128 : 64;
+  s.BN = 4096 / s.BM;
+  const int per_batch = pr.tiles_m * pr.tiles_n;
+  const int bz = w.y / per_batch;
+  const int r = w.y - bz * per_batch;
+  const int tm = r / pr.tiles_n, tn = r - tm * pr.tiles_n;
+  const int m0 = tm * s.BM, n0 = tn * s.BN;
+  s.mr = min(s.BM, pr.M - m0);
+  s.nr = min(s.BN, pr.N - n0);
+  s.K = pr.K;
+  s.A = pr.A;
+  s.B = pr.B;
+  s.C = pr.C;
+  s.a_kmajor = pr.a_kmajor;
+  s.b_kmajor = pr.b_kmajor;
+  s.c_mfast = pr.c_mfast;
+  const int e = tid, BM = s.BM, BN = s.BN, K = s.K;
+  if (e < BM) {
+    soff[SOFF_ROWA + e] = e < s.mr ? pr.rowA[m0 + e] : 0;
+    soff[SOFF_ROWC + e] = e < s.mr ? pr.rowC[m0 + e] : 0;
+  } else if (e < BM + BN) {
+    const int n = e - BM;
+    soff[SOFF_COLB + n] = n < s.nr ? pr.colB[n0 + n] : 0;
+    soff[SOFF_COLC + n] = n < s.nr ? pr.colC[n0 + n] : 0;
+  } else if (e < BM + BN + K) {
+    const int k = e - BM - BN;
+    soff[SOFF_KA + k] = pr.kA[k];
+    soff[SOFF_KB + k] = pr.kB[k];
+  } else if (e < BM + BN + K + 3) {
+    const int j = e - BM - BN - K;
+    soff[SOFF_BATCH + j] = j == 0 ? pr.bA[bz] : j == 1 ? pr.bB[bz] : pr.bC[bz];
+  }
+  return s;
+}
+
+// Gathers the K pass starting at k0: at most SGA elements of A and SGA of B
+// per thread, all loads issued before any is used.
+template <typename T, bool CP>
+__device__ __forceinline__ void shallow_gather(const ShallowItem& s, const int64_t* soff, int k0, Cplx<T>* ra,
+                                               Cplx<T>* rb) {
+  const int tid = threadIdx.x;
+  const int kn = min(SKC, s.K - k0);
   const int64_t offA = soff[SOFF_BATCH], offB = soff[SOFF_BATCH + 1];
-  // this thread's line of C: fixed index f along C's contiguous direction,
-  // SOUT indices v0 + vs * i across it
-  const int LF = CM ? BM : BN;
-  const int f = tid & (LF - 1), v0 = tid / LF, vs = SNT / LF;
-  Cplx<T> acc[SOUT];
 #pragma unroll
-  for (int i = 0; i < SOUT; ++i) acc[i] = Cplx<T>{T(0), T(0)};
-  const int K = pr.K;
-  for (int k0 = 0; k0 < K; k0 += SKC) {
-    const int kn = min(SKC, K - k0);
-    if (k0 > 0) __syncthreads();  // readers of the previous pass are done
-    // gather: at most SGA elements of A and SGA of B per thread, all loads first
-    Cplx<T> ra[SGA], rb[SGA];
-#pragma unroll
-    for (int j = 0; j < SGA; ++j) {
-      const int e = tid + SNT * j;
-      ra[j] = rb[j] = Cplx<T>{T(0), T(0)};
-      if (e < BM * SKC) {
-        const int m = pr.a_kmajor ? e / SKC : e & (BM - 1), k = pr.a_kmajor ? e & (SKC - 1) : e / BM;
-        if (m < mr && k < kn) ra[j] = load_elem<T, CP>(pr.A, offA + soff[SOFF_ROWA + m] + soff[SOFF_KA + k0 + k]);
-      }
-      if (e < BN * SKC) {
-        const int n = pr.b_kmajor ? e / SKC : e & (BN - 1), k = pr.b_kmajor ? e & (SKC - 1) : e / BN;
-        if (n < nr && k < kn) rb[j] = load_elem<T, CP>(pr.B, offB + soff[SOFF_COLB + n] + soff[SOFF_KB + k0 + k]);
-      }
+  for (int j = 0; j < SGA; ++j) {
+    const int e = tid + SNT * j;
+    ra[j] = rb[j] = Cplx<T>{T(0), T(0)};
+    if (e < s.BM * SKC) {
+      const int m = s.a_kmajor ? e / SKC : e & (s.BM - 1), k = s.a_kmajor ? e & (SKC - 1) : e / s.BM;
+      if (m < s.mr && k < kn) ra[j] = load_elem<T, CP>(s.A, offA + soff[SOFF_ROWA + m] + soff[SOFF_KA + k0 + k]);
     }
-#pragma unroll
-    for (int j = 0; j < SGA; ++j) {
-      const int e = tid + SNT * j;
-      if (e < BM * SKC) {
-        const int m = pr.a_kmajor ? e / SKC : e & (BM - 1), k = pr.a_kmajor ? e & (SKC - 1) : e / BM;
-        sA[k * BM + m] = ra[j];
-      }
-      if (e < BN * SKC) {
-        const int n = pr.b_kmajor ? e / SKC : e & (BN - 1), k = pr.b_kmajor ? e & (SKC - 1) : e / BN;
-        sB[k * BN + n] = rb[j];
-      }
-    }
-    __syncthreads();
-    for (int kk = 0; kk < kn; ++kk) {
-      if constexpr (CM) {
-        const Cplx<T> a = sA[kk * BM + f];
-#pragma unroll
-        for (int i = 0; i < SOUT; ++i) cmac<T, CP>(acc[i], a, sB[kk * BN + v0 + vs * i]);
-      } else {
-        const Cplx<T> b = sB[kk * BN + f];
-#pragma unroll
-        for (int i = 0; i < SOUT; ++i) cmac<T, CP>(acc[i], sA[kk * BM + v0 + vs * i], b);
-      }
+    if (e < s.BN * SKC) {
+      const int n = s.b_kmajor ? e / SKC : e & (s.BN - 1), k = s.b_kmajor ? e & (SKC - 1) : e / s.BN;
+      if (n < s.nr && k < kn) rb[j] = load_elem<T, CP>(s.B, offB + soff[SOFF_COLB + n] + soff[SOFF_KB + k0 + k]);
     }
   }
+}
+
+// Writes a gathered pass into LDS, k-major ([k][m] for A, [k][n] for B).
+template <typename T>
+__device__ __forceinline__ void shallow_stash(const ShallowItem& s, Cplx<T>* sab, const Cplx<T>* ra,
+                                              const Cplx<T>* rb) {
+  const int tid = threadIdx.x;
+  Cplx<T>* sA = sab;
+  Cplx<T>* sB = sab + s.BM * SKC;
+#pragma unroll
+  for (int j = 0; j < SGA; ++j) {
+    const int e = tid + SNT * j;
+    if (e < s.BM * SKC) {
+      const int m = s.a_kmajor ? e / SKC : e & (s.BM - 1), k = s.a_kmajor ? e & (SKC - 1) : e / s.BM;
+      sA[k * s.BM + m] = ra[j];
+    }
+    if (e < s.BN * SKC) {
+      const int n = s.b_kmajor ? e / SKC : e & (s.BN - 1), k = s.b_kmajor ? e & (SKC - 1) : e / s.BN;
+      sB[k * s.BN + n] = rb[j];
+    }
+  }
+}
+
+// Accumulates kn staged k of this thread's line of C: fixed index f along C's
+// contiguous direction, SOUT indices v0 + vs * i across it.
+template <typename T, bool CP, bool CM>
+__device__ __forceinline__ void shallow_compute(const ShallowItem& s, const Cplx<T>* sab, int kn, Cplx<T>* acc) {
+  const int tid = threadIdx.x;
+  const Cplx<T>* sA = sab;
+  const Cplx<T>* sB = sab + s.BM * SKC;
+  const int LF = CM ? s.BM : s.BN;
+  const int f = tid & (LF - 1), v0 = tid / LF, vs = SNT / LF;
+  for (int kk = 0; kk < kn; ++kk) {
+    if constexpr (CM) {
+      const Cplx<T> av = sA[kk * s.BM + f];
+#pragma unroll
+      for (int i = 0; i < SOUT; ++i) cmac<T, CP>(acc[i], av, sB[kk * s.BN + v0 + vs * i]);
+    } else {
+      const Cplx<T> bv = sB[kk * s.BN + f];
+#pragma unroll
+      for (int i = 0; i < SOUT; ++i) cmac<T, CP>(acc[i], sA[kk * s.BM + v0 + vs * i], bv);
+    }
+  }
+}
+
+template <typename T, bool CP, bool CM>
+__device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_t* soff, const Cplx<T>* acc) {
+  const int tid = threadIdx.x;
+  const int LF = CM ? s.BM : s.BN;
+  const int f = tid & (LF - 1), v0 = tid / LF, vs = SNT / LF;
   const int64_t offC = soff[SOFF_BATCH + 2];
 #pragma unroll
   for (int i = 0; i < SOUT; ++i) {
     const int m = CM ? f : v0 + vs * i, n = CM ? v0 + vs * i : f;
-    if (m < mr && n < nr) store_elem<T, CP>(pr.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
+    if (m < s.mr && n < s.nr) store_elem<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
   }
 }
 
-// 6 waves per SIMD: three 8-wave workgroups per CU (80 registers, no spills)
-// 6 waves per SIMD: three 8-wave workgroups per CU (80 registers, no spills)
-template <typename T, bool CP>
-__global__ __launch_bounds__(SNT) __attribute__((amdgpu_waves_per_eu(6, 8))) void shallow_kernel(
+// Two 8-wave workgroups per CU (about 126 registers for complex FP64: the
+// next item's gathered pass is held across the stores); one launch per C
+// orientation (CM: C runs along M).  Each workgroup loops over items; the
+// next item's table phase and first gather are issued before the current
+// item's stores, so its loads are not queued behind them (vector memory
+// operations complete in order) and the stores drain while the next item is
+// computed.  soff is double-buffered (the stores read the current item's C
+// offsets after the next item's table phase).
+template <typename T, bool CP, bool CM>
+__global__ __launch_bounds__(SNT) void shallow_kernel(
     ShallowArgs a) {
   __shared__ Cplx<T> sab[SMAXE * SKC];
-  __shared__ int64_t soff[SOFF_N];
-  const int tid = threadIdx.x;
-  for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
-    const int2 w = a.items[it];
-    const BatchPair& pr = a.pairs[w.x];
-    const int BM = pr.tile == MUSCLE_HIP_TILE_32x128 ? 32 : pr.tile == MUSCLE_HIP_TILE_128x32 ? 128 : 64;
-    const int BN = 4096 / BM;
-    const int per_batch = pr.tiles_m * pr.tiles_n;
-    const int bz = w.y / per_batch;
-    const int r = w.y - bz * per_batch;
-    const int tm = r / pr.tiles_n, tn = r - tm * pr.tiles_n;
-    const int m0 = tm * BM, n0 = tn * BN;
-    const int mr = min(BM, pr.M - m0), nr = min(BN, pr.N - n0), K = pr.K;
-    __syncthreads();  // the previous item's readers of LDS are done
-    // one table lookup per thread: rows, columns, K offsets, batch offsets
-    const int e = tid;
-    if (e < BM) {
-      soff[SOFF_ROWA + e] = e < mr ? pr.rowA[m0 + e] : 0;
-      soff[SOFF_ROWC + e] = e < mr ? pr.rowC[m0 + e] : 0;
-    } else if (e < BM + BN) {
-      const int n = e - BM;
-      soff[SOFF_COLB + n] = n < nr ? pr.colB[n0 + n] : 0;
-      soff[SOFF_COLC + n] = n < nr ? pr.colC[n0 + n] : 0;
-    } else if (e < BM + BN + K) {
-      const int k = e - BM - BN;
-      soff[SOFF_KA + k] = pr.kA[k];
-      soff[SOFF_KB + k] = pr.kB[k];
-    } else if (e < BM + BN + K + 3) {
-      const int j = e - BM - BN - K;
-      soff[SOFF_BATCH + j] = j == 0 ? pr.bA[bz] : j == 1 ? pr.bB[bz] : pr.bC[bz];
+  __shared__ int64_t soff[2][SOFF_N];
+  int64_t it = blockIdx.x;
+  if (it >= a.nitems) return;
+  int b = 0;
+  ShallowItem cur = shallow_tables(a, it, soff[0]);
+  __syncthreads();
+  Cplx<T> ra[SGA], rb[SGA];
+  shallow_gather<T, CP>(cur, soff[0], 0, ra, rb);
+  for (;;) {
+    Cplx<T> acc[SOUT];
+#pragma unroll
+    for (int i = 0; i < SOUT; ++i) acc[i] = Cplx<T>{T(0), T(0)};
+    for (int k0 = 0;;) {
+      __syncthreads();  // the readers of sab (and of soff[b ^ 1], by the previous item) are done
+      shallow_stash<T>(cur, sab, ra, rb);
+      __syncthreads();
+      shallow_compute<T, CP, CM>(cur, sab, min(SKC, cur.K - k0), acc);
+      k0 += SKC;
+      if (k0 >= cur.K) break;
+      __syncthreads();
+      shallow_gather<T, CP>(cur, soff[b], k0, ra, rb);
     }
-    __syncthreads();
-    if (pr.c_mfast) shallow_tile<T, CP, true>(pr, BM, BN, mr, nr, sab, soff);
-    else shallow_tile<T, CP, false>(pr, BM, BN, mr, nr, sab, soff);
+    const int64_t it2 = it + gridDim.x;
+    const bool more = it2 < a.nitems;
+    ShallowItem nxt;
+    if (more) {
+      nxt = shallow_tables(a, it2, soff[b ^ 1]);
+      __syncthreads();
+      shallow_gather<T, CP>(nxt, soff[b ^ 1], 0, ra, rb);
+    }
+    shallow_store<T, CP, CM>(cur, soff[b], acc);
+    if (!more) break;
+    cur = nxt;
+    b ^= 1;
+    it = it2;
   }
 }
 
@@ -626,7 +696,7 @@ __device__ __forceinline__ void skinny_tile(const BatchPair& pr, int bz, int tl,
 // One launch per J (the items of pairs with the same long positions per
 // thread), so each has the registers its own ring and accumulators need.
 template <typename T, bool CP, int J>
-__global__ __launch_bounds__(KNT) __attribute__((amdgpu_waves_per_eu(6, 8))) void skinny_kernel(ShallowArgs a) {
+__global__ __launch_bounds__(KNT) void skinny_kernel(ShallowArgs a) {
   __shared__ Cplx<T> sS[KSB];
   __shared__ int64_t sK[KOFF_N];
   for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
@@ -676,17 +746,27 @@ size_t batch_lds_bytes(bool dbl, bool cplx, int tile) {
   return bufs * pl * BK * ((size_t)(t.bm + LDP) + (t.bn + LDP)) * el;
 }
 
-void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s) {
+void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStream_t s) {
   if (a.nitems <= 0) return;
   // a grid-stride loop over the items; 2^20 workgroups fill the GPU many times
   const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, int64_t(1) << 20);
+#define MH_SHALLOW(T, CP)                                                                           \
+  if (cm) hipLaunchKernelGGL((shallow_kernel<T, CP, true>), dim3(grid), dim3(SNT), 0, s, a);       \
+  else hipLaunchKernelGGL((shallow_kernel<T, CP, false>), dim3(grid), dim3(SNT), 0, s, a);
   if (dbl) {
-    if (cplx) hipLaunchKernelGGL((shallow_kernel<double, true>), dim3(grid), dim3(SNT), 0, s, a);
-    else hipLaunchKernelGGL((shallow_kernel<double, false>), dim3(grid), dim3(SNT), 0, s, a);
+    if (cplx) {
+      MH_SHALLOW(double, true)
+    } else {
+      MH_SHALLOW(double, false)
+    }
   } else {
-    if (cplx) hipLaunchKernelGGL((shallow_kernel<float, true>), dim3(grid), dim3(SNT), 0, s, a);
-    else hipLaunchKernelGGL((shallow_kernel<float, false>), dim3(grid), dim3(SNT), 0, s, a);
+    if (cplx) {
+      MH_SHALLOW(float, true)
+    } else {
+      MH_SHALLOW(float, false)
+    }
   }
+#undef MH_SHALLOW
 }
 
 void launch_skinny(bool dbl, bool cplx, int j, const ShallowArgs& a, hipStream_t s) {

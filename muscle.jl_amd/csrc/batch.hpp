@@ -91,8 +91,9 @@ struct ShallowArgs {
   int64_t nitems;
 };
 
-// Launches the shallow kernel over a.nitems tiles (nothing when there are none).
-void launch_shallow(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s);
+// Launches the shallow kernel over a.nitems tiles of pairs whose C runs along
+// M (cm) or N (nothing when there are none).
+void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStream_t s);
 // Launches the skinny kernel for pairs of j long positions per thread over
 // a.nitems items (nothing when there are none).
 void launch_skinny(bool dbl, bool cplx, int j, const ShallowArgs& a, hipStream_t s);
@@ -129,6 +130,7 @@ struct Batch {
   int64_t nskinny = 0;   // the skinny items at the end of hitems
   int64_t nskinny_j[3] = {0, 0, 0};  // of them, those with j = 1, 2, 4 (in that order)
   int64_t nshallow = 0;  // the shallow items before them
+  int64_t nshallow_cm[2] = {0, 0};  // of them, those of pairs with C along N, then along M
   int64_t nsplit = 0;    // the slice items before them
   // split K (per pair of the work list, by position): slices, K per slice,
   // offsets of the pair's partial tiles (bytes) and arrival counters

@@ -165,7 +165,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     q.k = L.d.K;
     q.batch = L.d.B;
   }
-  std::vector<int2> shallow, sliced, skinny[3];
+  std::vector<int2> shallow[2], sliced, skinny[3];
   const int sko = skinny_outputs(is_double(B.dtype), cp);
   for (int64_t i : order) {
     const PlannedPair& L = *lw[(size_t)i];
@@ -211,14 +211,20 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     } else if (ks > 1) {
       B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SPLIT_K;
     }
-    std::vector<int2>& dst = sk ? skinny[sg.j == 1 ? 0 : sg.j == 2 ? 1 : 2] : sh ? shallow : ks > 1 ? sliced : B.hitems;
+    std::vector<int2>& dst = sk   ? skinny[sg.j == 1 ? 0 : sg.j == 2 ? 1 : 2]
+                             : sh ? shallow[L.c_mfast ? 1 : 0]
+                             : ks > 1 ? sliced
+                                      : B.hitems;
     for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
     if (!sh && !sk) used[L.tile] = true;
   }
   B.nsplit = (int64_t)sliced.size();
-  B.nshallow = (int64_t)shallow.size();
+  B.nshallow = (int64_t)(shallow[0].size() + shallow[1].size());
   B.hitems.insert(B.hitems.end(), sliced.begin(), sliced.end());
-  B.hitems.insert(B.hitems.end(), shallow.begin(), shallow.end());
+  for (int c = 0; c < 2; ++c) {
+    B.nshallow_cm[c] = (int64_t)shallow[c].size();
+    B.hitems.insert(B.hitems.end(), shallow[c].begin(), shallow[c].end());
+  }
   B.nskinny = 0;
   for (int j = 0; j < 3; ++j) {
     B.nskinny_j[j] = (int64_t)skinny[j].size();
@@ -401,11 +407,18 @@ void batch_launch(Batch& b, hipStream_t s) {
     launch_batch(is_double(b.dtype), is_complex(b.dtype), a, sp, b.workgroups, lds, s);
   }
   if (b.nshallow > 0) {
-    ShallowArgs a;
-    a.pairs = b.dpairs;
-    a.items = b.ditems + nmain + b.nsplit;
-    a.nitems = b.nshallow;
-    launch_shallow(is_double(b.dtype), is_complex(b.dtype), a, s);
+    // one launch per C orientation
+    int64_t off = nmain + b.nsplit;
+    for (int c = 0; c < 2; ++c) {
+      if (b.nshallow_cm[c] > 0) {
+        ShallowArgs a;
+        a.pairs = b.dpairs;
+        a.items = b.ditems + off;
+        a.nitems = b.nshallow_cm[c];
+        launch_shallow(is_double(b.dtype), is_complex(b.dtype), c == 1, a, s);
+      }
+      off += b.nshallow_cm[c];
+    }
   }
   // the skinny items, one launch per long positions per thread (1, 2, 4)
   int64_t off = nmain + b.nsplit + b.nshallow;
