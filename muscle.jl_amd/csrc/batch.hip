@@ -281,7 +281,7 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
 // not a memset, so that it can be graph captured).
 __global__ void reset_counter_kernel(int32_t* c, int32_t* tilecnt, int64_t n) {
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i0 == 0) *c = 0;
+  if (i0 < kXcds) c[i0] = 0;
   for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) tilecnt[i] = 0;
 }
 
@@ -292,12 +292,26 @@ __global__ __launch_bounds__(NT, kBatchWgPerCu) void batch_kernel(BatchArgs a) {
   // the broadcast word lives in dynamic LDS no tile shape of the launch
   // touches (see batch_item_offset)
   int* s_item = reinterpret_cast<int*>(smem_raw + a.item_off);
+  // this workgroup's XCD queue first; once it is drained, the others in turn
+  const int x0 = blockIdx.x % kXcds;
+  int q = 0;
   for (;;) {
-    if (threadIdx.x == 0) *s_item = atomicAdd(a.counter, 1);
+    if (threadIdx.x == 0) {
+      int item = -1;
+      for (; q < kXcds; ++q) {
+        const int x = (x0 + q) % kXcds;
+        const int i = a.qoff[x] + atomicAdd(&a.counter[x], 1);
+        if (i < a.qoff[x + 1]) {
+          item = i;
+          break;
+        }
+      }
+      *s_item = item;
+    }
     __syncthreads();
     const int64_t it = *s_item;
     __syncthreads();  // every thread holds the item before thread 0 fetches the next
-    if (it >= a.nitems) break;
+    if (it < 0) break;
     const int2 w = a.items[it];
     const BatchPair& pr = a.pairs[w.x];
     const int per_batch = pr.tiles_m * pr.tiles_n;

@@ -36,10 +36,17 @@ struct TileShape {
 };
 constexpr TileShape kTileShapes[MUSCLE_HIP_TILE_COUNT] = {{64, 64, 2, 2}, {32, 128, 1, 4}, {128, 32, 4, 1}};
 
+// The persistent kernel's work list is split into one queue per XCD (the
+// dispatcher deals workgroups round-robin over the XCDs, so workgroup b runs
+// on XCD b % kXcds): each pair's tiles are cut into kXcds contiguous runs, one
+// per queue, so the tiles that share A and B panels meet in one L2.
+constexpr int kXcds = 8;
+
 struct BatchArgs {
   const BatchPair* pairs;
-  const int2* items;  // (pair, tile within pair), largest pairs first
-  int32_t* counter;   // work queue head
+  const int2* items;  // (pair, tile within pair), per queue largest pairs first
+  int32_t* counter;   // work queue heads (kXcds of them)
+  int32_t qoff[kXcds + 1];  // queue q holds items [qoff[q], qoff[q + 1])
   int32_t* tilecnt;   // arrival counters of the split pairs' tiles, cleared with the queue head
   int64_t ntilecnt;
   int64_t nitems;
@@ -127,6 +134,7 @@ struct Batch {
   // persistent items (largest K first), then the split pairs' slice items,
   // then the shallow items, then the skinny items
   std::vector<int2> hitems;
+  int32_t qoff[kXcds + 1] = {0};  // the persistent items' per-XCD queues (offsets into hitems)
   int64_t nskinny = 0;   // the skinny items at the end of hitems
   int64_t nskinny_j[3] = {0, 0, 0};  // of them, those with j = 1, 2, 4 (in that order)
   int64_t nshallow = 0;  // the shallow items before them

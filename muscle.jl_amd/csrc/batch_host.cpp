@@ -165,7 +165,8 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     q.k = L.d.K;
     q.batch = L.d.B;
   }
-  std::vector<int2> shallow[2], sliced, skinny[3];
+  std::vector<int2> shallow[2], sliced, skinny[3], xq[kXcds];
+  double xload[kXcds] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int sko = skinny_outputs(is_double(B.dtype), cp);
   for (int64_t i : order) {
     const PlannedPair& L = *lw[(size_t)i];
@@ -211,12 +212,34 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     } else if (ks > 1) {
       B.pinfo[(size_t)i].route = MUSCLE_HIP_PAIR_SPLIT_K;
     }
-    std::vector<int2>& dst = sk   ? skinny[sg.j == 1 ? 0 : sg.j == 2 ? 1 : 2]
-                             : sh ? shallow[L.c_mfast ? 1 : 0]
-                             : ks > 1 ? sliced
-                                      : B.hitems;
-    for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
+    if (!sk && !sh && ks == 1) {
+      // persistent tiles: kXcds contiguous runs, one per XCD queue; a pair of
+      // fewer tiles goes whole to the queue with the least work so far
+      const double cost = (double)L.cost;
+      if (items >= kXcds) {
+        for (int x = 0; x < kXcds; ++x) {
+          const int64_t t0 = items * x / kXcds, t1 = items * (x + 1) / kXcds;
+          for (int64_t t = t0; t < t1; ++t) xq[x].push_back(make_int2((int)i, (int)t));
+          xload[x] += cost * (double)(t1 - t0);
+        }
+      } else {
+        const int x = (int)(std::min_element(xload, xload + kXcds) - xload);
+        for (int64_t t = 0; t < items; ++t) xq[x].push_back(make_int2((int)i, (int)t));
+        xload[x] += cost * (double)items;
+      }
+    } else {
+      std::vector<int2>& dst = sk ? skinny[sg.j == 1 ? 0 : sg.j == 2 ? 1 : 2] : sh ? shallow[L.c_mfast ? 1 : 0] : sliced;
+      for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
+    }
     if (!sh && !sk) used[L.tile] = true;
+  }
+  size_t nq = 0;
+  for (int x = 0; x < kXcds; ++x) nq += xq[x].size();
+  if (nq >= (size_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one batch");
+  B.qoff[0] = 0;
+  for (int x = 0; x < kXcds; ++x) {
+    B.hitems.insert(B.hitems.end(), xq[x].begin(), xq[x].end());
+    B.qoff[x + 1] = (int32_t)B.hitems.size();
   }
   B.nsplit = (int64_t)sliced.size();
   B.nshallow = (int64_t)(shallow[0].size() + shallow[1].size());
@@ -395,6 +418,7 @@ void batch_launch(Batch& b, hipStream_t s) {
     a.pairs = b.dpairs;
     a.items = b.ditems;
     a.counter = b.dcounter;
+    for (int x = 0; x <= kXcds; ++x) a.qoff[x] = b.qoff[x];
     a.tilecnt = b.dtilecnt;
     a.ntilecnt = b.ntilecnt;
     a.nitems = nmain;
