@@ -55,6 +55,10 @@ def issued_mfma(precision: str, complex_mult: int, slices: int, mnkb: float, ker
     peak = {"bf16x3": "bf16", "fp16x2": "fp16"}.get(precision, "int8")
     return 3 * pairs * 2.0 * mnkb, PEAK_TFLOPS[peak]
 HBM_TBS = 8.0
+# HBM rates this library's own streaming kernels reach on the device
+# (muscle.measure_bandwidth: 16-byte stores, then loads), set by main() when a
+# write- or read-bound section runs; the second ceiling such sections report
+MEASURED_BW = None
 METRIC = "binary_einsum TFLOP/s (% MFMA peak) vs plain at stated error; jobs/s + p99 latency"
 
 
@@ -197,11 +201,14 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
     nbytes = sum(i["bytes"] for i in infos)
     # per-pair roofline: max(flops / MFMA peak, bytes / HBM bandwidth), summed
     peak = (PEAK_TFLOPS["fp64"] if dtype == "c64" else PEAK_TFLOPS["fp32"]) * 1e12
-    roof_s = 0.0
+    roof_s = roof_m = 0.0
     for ia, ib, ic in pairs:
         fl = 8.0 * 2.0 ** len(set(ia) | set(ib))
         by = esz * (2.0 ** len(ia) + 2.0 ** len(ib) + 2.0 ** len(ic))
         roof_s += max(fl / peak, by / (HBM_TBS * 1e12))
+        if MEASURED_BW:  # operands read at the measured read rate, C written at the measured write rate
+            rd, wr = esz * (2.0 ** len(ia) + 2.0 ** len(ib)), esz * 2.0 ** len(ic)
+            roof_m += max(fl / peak, rd / (MEASURED_BW["read_tbs"] * 1e12) + wr / (MEASURED_BW["write_tbs"] * 1e12))
 
     def run_all(mode):
         for bt in batches:
@@ -225,6 +232,8 @@ def bench_batched(dtype: str, n: int, seed: int, budget_gb: float, steps: int, l
         "loop_ms": round(ms_l, 3), "loop_pairs_per_s": round(n / (ms_l / 1e3), 1),
         "speedup_vs_plain_loop": round(ms_l / ms_p, 3),
         "roofline_ms": round(roof_s * 1e3, 3), "roofline_frac": round(roof_s * 1e3 / ms_p, 4),
+        "roofline_measured_bw_ms": round(roof_m * 1e3, 3) if MEASURED_BW else None,
+        "roofline_measured_bw_frac": round(roof_m * 1e3 / ms_p, 4) if MEASURED_BW else None,
         "total_tflop": round(flops / 1e12, 3), "total_gb": round(nbytes / 1e9, 2),
         "autotune_s": round(tune_s, 2), "persistent_vs_loop_max_rel_diff": diff,
     }
@@ -356,19 +365,24 @@ def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, thre
     # and the HBM time of its operands and output (every element once, 8 TB/s)
     def labels(s, x):
         return set(s.leaves[x]) if x < len(s.leaves) else set(s.nodes[x - len(s.leaves)][2])
-    t_mfma, t_roof = [], []
+    t_mfma, t_roof, t_meas = [], [], []
     for s in specs:
         peak = (PEAK_TFLOPS["fp64"] if s.dtype == "c64" else PEAK_TFLOPS["fp32"]) * 1e12
         es = 16 if s.dtype == "c64" else 8
-        tm = tr = 0.0
+        tm = tr = tq = 0.0
         for a, b, o in s.nodes:
             la, lb = labels(s, a), labels(s, b)
             f = 6.0 * 2 ** len(la | lb) / peak
             by = (2 ** len(la) + 2 ** len(lb) + 2 ** len(set(o))) * es / (HBM_TBS * 1e12)
             tm += f
             tr += max(f, by)
+            if MEASURED_BW:
+                q = ((2 ** len(la) + 2 ** len(lb)) * es / (MEASURED_BW["read_tbs"] * 1e12) +
+                     2 ** len(set(o)) * es / (MEASURED_BW["write_tbs"] * 1e12))
+                tq += max(f, q)
         t_mfma.append(tm)
         t_roof.append(tr)
+        t_meas.append(tq)
     res = {"distribution": f"trees of 20-300 binary_einsums, dims 2, tensor sizes log-uniform in 2^{size[0]}.."
                            f"2^{size[1]}, ComplexF32/ComplexF64 per job",
            "jobs": n_jobs, "c64_jobs": sum(s.dtype == "c64" for s in specs),
@@ -376,6 +390,7 @@ def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, thre
            "leaf_bank_gib": round(bank.bytes() / 2 ** 30, 2), "pool_gib": round(pool / 2 ** 30, 2),
            "ceiling_jobs_per_s_mfma": round(n_jobs / sum(t_mfma), 2),
            "ceiling_jobs_per_s_roofline": round(n_jobs / sum(t_roof), 2),
+           "ceiling_jobs_per_s_measured_bw": round(n_jobs / sum(t_meas), 2) if MEASURED_BW else None,
            "ceiling_note": "jobs/s if every node ran at its bound with no launch or scheduling gaps: MFMA "
                            "(3M issued flops at the dtype's dense peak) or the larger of that and the node's "
                            "HBM bytes at 8 TB/s"}
@@ -404,6 +419,8 @@ def bench_served(dist: str, n_jobs: int, rates, seed: int, check_jobs: int, thre
     res["speedup_vs_plain_loop"] = round(best / serial["jobs_per_s"], 3)  # capacity over capacity
     res["served_frac_of_roofline_ceiling"] = round(best / res["ceiling_jobs_per_s_roofline"], 4)
     res["served_frac_of_mfma_ceiling"] = round(best / res["ceiling_jobs_per_s_mfma"], 4)
+    if MEASURED_BW:
+        res["served_frac_of_measured_bw_ceiling"] = round(best / res["ceiling_jobs_per_s_measured_bw"], 4)
     res["plain_loop_frac_of_roofline_ceiling"] = round(serial["jobs_per_s"] / res["ceiling_jobs_per_s_roofline"], 4)
     if keep and threads > 0:
         from threadpoolctl import threadpool_limits
@@ -743,6 +760,14 @@ def main() -> int:
                             "mean_elementwise": float(el.mean()), "us_per_call": round(m1 * 1e3, 2)}
             res[dist] = row
         extra["ladder_c1"] = res
+    if want("batched") or want("served") or want("memory_bound"):
+        global MEASURED_BW
+        try:
+            MEASURED_BW = muscle.measure_bandwidth(8.0)
+            extra["hbm_measured"] = {k: round(v, 3) for k, v in MEASURED_BW.items()}
+            log(f"HBM measured: {extra['hbm_measured']}")
+        except Exception as e:  # reported in the JSON, not hidden
+            extra["hbm_measured"] = {"error": repr(e)}
     if want("batched"):
         for bdt in args.batch_dtypes.split(","):
             try:

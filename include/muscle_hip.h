@@ -141,6 +141,14 @@ MUSCLE_HIP_API int muscle_hip_destroy(muscle_hip_handle_t handle);
  * still exist at this call.  Calls that must overlap use separate handles. */
 MUSCLE_HIP_API int muscle_hip_set_stream(muscle_hip_handle_t handle, void* stream);
 MUSCLE_HIP_API int muscle_hip_synchronize(muscle_hip_handle_t handle);
+/* Measures the HBM rates this library's kernels can reach on the handle's
+ * device: 16-byte stores, then loads, streamed over a fresh buffer of `bytes`
+ * (at least 1 MiB; allocated and freed here, synchronizes the handle's
+ * stream), five passes each, in TB/s.  The ceilings that write- and
+ * read-bound launches are measured against (the nominal 8 TB/s is not
+ * reached by either).  Replaces no reference function. */
+MUSCLE_HIP_API int muscle_hip_measure_bandwidth(muscle_hip_handle_t handle, int64_t bytes, double* read_tbs,
+                                                double* write_tbs);
 MUSCLE_HIP_API const char* muscle_hip_status_string(int status);
 /* Detail of the last failing call on this thread. */
 MUSCLE_HIP_API const char* muscle_hip_last_error(void);

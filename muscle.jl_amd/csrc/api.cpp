@@ -756,6 +756,43 @@ int muscle_hip_set_option(muscle_hip_handle_t handle, int option, int64_t value)
   });
 }
 
+int muscle_hip_measure_bandwidth(muscle_hip_handle_t handle, int64_t bytes, double* read_tbs, double* write_tbs) {
+  return guarded([&] {
+    Handle* h = reinterpret_cast<Handle*>(handle);
+    if (!h || !read_tbs || !write_tbs) fail(MUSCLE_HIP_INVALID_ARGUMENT, "null handle or result pointer");
+    if (bytes < (int64_t(1) << 20)) fail(MUSCLE_HIP_INVALID_ARGUMENT, "probe buffer below 1 MiB");
+    bytes &= ~int64_t(15);
+    std::lock_guard<std::mutex> lk(h->mu);
+    void* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)bytes + 256) != hipSuccess) {
+      (void)hipGetLastError();
+      fail(MUSCLE_HIP_OUT_OF_MEMORY, "probe buffer allocation failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double rate[2] = {0, 0};
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    for (int w = 1; w >= 0 && err == hipSuccess; --w) {  // write first: the read pass reads written memory
+      uint32_t* out = reinterpret_cast<uint32_t*>((char*)buf + bytes);
+      launch_probe(w == 1, buf, bytes, out, h->stream);
+      err = hipEventRecord(e0, h->stream);
+      for (int r = 0; r < 5 && err == hipSuccess; ++r) launch_probe(w == 1, buf, bytes, out, h->stream);
+      if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err == hipSuccess) rate[w] = 5.0 * (double)bytes / (ms * 1e-3) / 1e12;
+    }
+    if (err == hipSuccess) err = hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    if (err != hipSuccess) fail(MUSCLE_HIP_HIP_ERROR, std::string("bandwidth probe: ") + hipGetErrorString(err));
+    *read_tbs = rate[0];
+    *write_tbs = rate[1];
+  });
+}
+
 int muscle_hip_synchronize(muscle_hip_handle_t handle) {
   return guarded([&] {
     Handle* h = reinterpret_cast<Handle*>(handle);

@@ -183,6 +183,33 @@ void launch_hadamard(int dtA, int dtB, const EwArgs& e, hipStream_t s) {
   }
 }
 
+// ------------------------------------------------------------------ bandwidth probe
+// Streams over a buffer with 16-byte accesses (grid-stride): the write kernel
+// stores a constant, the read kernel folds what it reads into one word that it
+// stores only when it matches a value no buffer written here holds, so the
+// loads are kept.  They give the HBM rates a kernel of this library can reach,
+// the ceilings its write- and read-bound launches are measured against.
+__global__ __launch_bounds__(256) void probe_write_kernel(uint4* p, int64_t n, uint32_t v) {
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) p[i] = make_uint4(v, v, v, v);
+}
+__global__ __launch_bounds__(256) void probe_read_kernel(const uint4* p, int64_t n, uint32_t* out) {
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  uint32_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) {
+    const uint4 x = p[i];
+    acc ^= x.x ^ x.y ^ x.z ^ x.w;
+  }
+  if (acc == 0x9e3779b9u) out[0] = acc;
+}
+
+void launch_probe(bool write, void* buf, int64_t bytes, uint32_t* out, hipStream_t s) {
+  const int64_t n = bytes / 16;
+  const dim3 g(32768), b(256);
+  if (write) hipLaunchKernelGGL(probe_write_kernel, g, b, 0, s, (uint4*)buf, n, 0u);
+  else hipLaunchKernelGGL(probe_read_kernel, g, b, 0, s, (const uint4*)buf, n, out);
+}
+
 void launch_cast(int dt_src, int dt_dst, const void* src, void* dst, int64_t n, hipStream_t s) {
   int64_t blocks = (n + NT - 1) / NT;
   if (blocks > 65536) blocks = 65536;

@@ -116,6 +116,9 @@ void launch_hadamard(int dtA, int dtB, const EwArgs& e, hipStream_t s);
 // ---- dtype cast over a strided span (mixed-precision operands) --------------
 // dst[i] = (dst type) src[i] for i in [0, n); complexness is preserved.
 void launch_cast(int dt_src, int dt_dst, const void* src, void* dst, int64_t n, hipStream_t s);
+// Bandwidth probe: streams `bytes` (a multiple of 16) of buf with 16-byte
+// stores (write) or loads (else; out receives at most one word).
+void launch_probe(bool write, void* buf, int64_t bytes, uint32_t* out, hipStream_t s);
 
 // Copies a GEMM operand from its strided layout into [batch][rows][K] with K
 // contiguous (rows = M for A, N for B); source offsets come from the plan's

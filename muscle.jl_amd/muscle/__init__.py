@@ -12,8 +12,8 @@ from .backend import (Backend, BackendBase, BackendCuTENSOR, BackendDagger, Back
                       BackendOMEinsum, BackendReactant, BackendStrided, Domain, DomainHost, DomainROCm,
                       choose_backend, choose_backend_rule, domain, with_backend)
 from .errors import ArgumentError, DimensionMismatch, MuscleHipError, UnsupportedError
-from .ops import (binary_einsum, binary_einsum_, hadamard, hadamard_, last_info, nonunique, unary_einsum,
-                  unary_einsum_)
+from .ops import (binary_einsum, binary_einsum_, hadamard, hadamard_, last_info, measure_bandwidth, nonunique,
+                  unary_einsum, unary_einsum_)
 from .tensor import Index, Tensor, col_major_strides, empty_col_major, inds, parent
 from .describe import describe
 from .factor import (AbsorbBehavior, AbsorbEqually, AbsorbU, AbsorbV, DontAbsorb, factorinds, simple_update,
@@ -25,6 +25,6 @@ __all__ = [
     "unary_einsum", "unary_einsum_", "nonunique", "Backend", "BackendBase", "BackendHIP", "BackendOMEinsum",
     "BackendDefault", "BackendStrided", "BackendCuTENSOR", "BackendReactant", "BackendDagger", "Domain",
     "DomainHost", "DomainROCm", "domain", "with_backend", "choose_backend", "choose_backend_rule",
-    "ArgumentError", "DimensionMismatch", "UnsupportedError", "MuscleHipError", "last_info", "describe",
+    "ArgumentError", "DimensionMismatch", "UnsupportedError", "MuscleHipError", "last_info", "measure_bandwidth", "describe",
     "col_major_strides", "empty_col_major",
 ]

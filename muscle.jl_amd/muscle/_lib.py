@@ -152,6 +152,8 @@ def _load() -> ctypes.CDLL:
         "muscle_hip_destroy": ([H], ctypes.c_int),
         "muscle_hip_set_stream": ([H, ctypes.c_void_p], ctypes.c_int),
         "muscle_hip_synchronize": ([H], ctypes.c_int),
+        "muscle_hip_measure_bandwidth": ([H, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "muscle_hip_set_option": ([H, ctypes.c_int, ctypes.c_int64], ctypes.c_int),
         "muscle_hip_status_string": ([ctypes.c_int], ctypes.c_char_p),
         "muscle_hip_last_error": ([], ctypes.c_char_p),

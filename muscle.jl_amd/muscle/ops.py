@@ -135,6 +135,16 @@ def _handle_for(dev: int):
     return h
 
 
+def measure_bandwidth(gib: float = 4.0, device=None) -> dict:
+    """HBM rates (TB/s) this library's kernels reach on `device`: 16-byte
+    stores, then loads, streamed over a fresh buffer of `gib` GiB."""
+    dev = torch.cuda.current_device() if device is None else device
+    h = _handle_for(dev)
+    r, w = ctypes.c_double(), ctypes.c_double()
+    _lib.check(_lib.lib.muscle_hip_measure_bandwidth(h, int(gib * 2 ** 30), ctypes.byref(r), ctypes.byref(w)))
+    return {"read_tbs": r.value, "write_tbs": w.value, "buffer_gib": gib}
+
+
 def _record(info: _lib.Info) -> None:
     _last.info = info.as_dict()
 

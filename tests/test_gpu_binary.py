@@ -321,3 +321,12 @@ def test_contracted_extent_zero(T):
     c = muscle.Tensor(torch.full((3, 4), float("nan"), dtype=a.dtype, device="cuda"), ["i", "k"])
     muscle.binary_einsum_(c, a, b)
     assert np.array_equal(host(c), np.zeros((3, 4), dtype=T))
+
+
+def test_bandwidth_probe():
+    """The HBM probe reports positive rates below the nominal peak and rejects
+    a buffer under 1 MiB."""
+    r = muscle.measure_bandwidth(1.0)
+    assert 0.3 < r["read_tbs"] < 8.0 and 0.3 < r["write_tbs"] < 8.0, r
+    with pytest.raises(muscle.ArgumentError):
+        muscle.measure_bandwidth(0.5 / 1024)
