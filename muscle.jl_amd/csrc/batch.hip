@@ -72,7 +72,7 @@ __device__ __forceinline__ void split_tile_fixup(const BatchPair& pr, int bz, in
       acc.re += v.re;
       acc.im += v.im;
     }
-    store_elem<T, CP>(pr.C, offC + pr.rowC[m0 + m] + pr.colC[n0 + n], acc);
+    store_elem_nt<T, CP>(pr.C, offC + pr.rowC[m0 + m] + pr.colC[n0 + n], acc);
   }
 }
 
@@ -271,7 +271,8 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
           re = acc0[i][j][r];
           im = acc1[i][j][r];
         }
-        store_elem<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});
+        if constexpr (split) store_elem<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});  // read back soon
+        else store_elem_nt<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});
       }
     }
   if constexpr (split) split_tile_fixup<T, CP, BM, BN>(pr, bz, tm, tn, s_flag);
@@ -519,7 +520,7 @@ __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_
 #pragma unroll
   for (int i = 0; i < SOUT; ++i) {
     const int m = CM ? f : v0 + vs * i, n = CM ? v0 + vs * i : f;
-    if (m < s.mr && n < s.nr) store_elem<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
+    if (m < s.mr && n < s.nr) store_elem_nt<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
   }
 }
 
@@ -703,7 +704,7 @@ __device__ __forceinline__ void skinny_tile(const BatchPair& pr, int bz, int tl,
     const int64_t cl = offC + cL[l];
 #pragma unroll
     for (int i = 0; i < QM; ++i)
-      if (q0 + i < S) store_elem<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
+      if (q0 + i < S) store_elem_nt<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
   }
 }
 

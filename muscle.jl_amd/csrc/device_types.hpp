@@ -53,6 +53,18 @@ __device__ inline void store_elem(void* base, int64_t off, Cplx<T> v) {
   }
 }
 
+// The same with a nontemporal (streaming) store.
+template <typename T, bool C>
+__device__ inline void store_elem_nt(void* base, int64_t off, Cplx<T> v) {
+  if constexpr (C) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 w = {v.re, v.im};
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(base) + off);
+  } else {
+    __builtin_nontemporal_store(v.re, reinterpret_cast<T*>(base) + off);
+  }
+}
+
 // Tile order of the GEMM kernels: the dispatcher deals workgroups
 // round-robin over the 8 XCDs, so workgroup L is renumbered to give each XCD
 // a contiguous range; the range is walked in groups of 8 tile rows, row
