@@ -174,21 +174,23 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     B.flops += (cp ? 8.0 : 2.0) * (double)L.d.M * (double)L.d.N * (double)L.d.K * (double)L.d.B;
     if (L.solo) continue;
     const TileShape& ts = kTileShapes[L.tile];
-    // a pair with an output side of at most kSkinnyMaxS: items of the whole
-    // short side by skinny_geom's run of the long side, per batch index
-    const bool sk = skinny_pair(L.d.M, L.d.N, L.d.K);
-    const SkinnyGeom sg = skinny_geom(std::min(L.d.M, L.d.N), sko);
-    const int64_t items = sk ? (std::max(L.d.M, L.d.N) + sg.lt - 1) / sg.lt * L.d.B : tiles_of(L);
-    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
-    const bool sh = !sk && L.d.K <= kShallowK;
     // Split K for a pair of few tiles (at most kSplitMaxTiles) and a long K
     // (at least kSplitMinKTiles K-tiles): slices of at least 8 K-tiles, at
     // most 16 of them.  Such tiles would otherwise set a small launch's
     // length.  The rule reads the pair alone, so a pair's result does not
-    // depend on what else is in the launch.
-    int64_t ks = 1, kc = L.d.K;
+    // depend on what else is in the launch.  It takes precedence over the
+    // skinny kernel, whose items run all of K in one workgroup.
     const int64_t kt = (L.d.K + 15) / 16;
-    if (B.allow_split && !sh && !sk && kt >= kSplitMinKTiles && items <= kSplitMaxTiles) {
+    const bool split_k = B.allow_split && L.d.K > kShallowK && kt >= kSplitMinKTiles && tiles_of(L) <= kSplitMaxTiles;
+    // a pair with an output side of at most kSkinnyMaxS: items of the whole
+    // short side by skinny_geom's run of the long side, per batch index
+    const bool sk = !split_k && skinny_pair(L.d.M, L.d.N, L.d.K);
+    const SkinnyGeom sg = skinny_geom(std::min(L.d.M, L.d.N), sko);
+    const int64_t items = sk ? (std::max(L.d.M, L.d.N) + sg.lt - 1) / sg.lt * L.d.B : tiles_of(L);
+    if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
+    const bool sh = !sk && L.d.K <= kShallowK;
+    int64_t ks = 1, kc = L.d.K;
+    if (split_k) {
       ks = std::min<int64_t>(16, kt / 8);
       kc = (kt + ks - 1) / ks * 16;
       ks = (L.d.K + kc - 1) / kc;
