@@ -710,19 +710,30 @@ __device__ __forceinline__ void skinny_tile(const BatchPair& pr, int bz, int tl,
 
 // One launch per J (the items of pairs with the same long positions per
 // thread), so each has the registers its own ring and accumulators need.
-template <typename T, bool CP, int J>
+// One launch for every skinny pair: each item takes the variant of its pair's
+// long positions per thread (J = 1, 2 or 4; complex FP64 uses 1 and 2).
+template <typename T, bool CP>
 __global__ __launch_bounds__(KNT) void skinny_kernel(ShallowArgs a) {
+  constexpr int O = (CP && sizeof(T) == 8) ? 4 : 8;
   __shared__ Cplx<T> sS[KSB];
   __shared__ int64_t sK[KOFF_N];
   for (int64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
     const int2 w = a.items[it];
     const BatchPair& pr = a.pairs[w.x];
     const bool sm = pr.M <= pr.N;
-    const SkinnyGeom g = skinny_geom(sm ? pr.M : pr.N, (CP && sizeof(T) == 8) ? 4 : 8);
+    const SkinnyGeom g = skinny_geom(sm ? pr.M : pr.N, O);
     const int tiles = ((sm ? pr.N : pr.M) + g.lt - 1) / g.lt;
     const int bz = w.y / tiles, tl = w.y - bz * tiles;
-    if (sm) skinny_tile<T, CP, J, true>(pr, bz, tl, sS, sK);
-    else skinny_tile<T, CP, J, false>(pr, bz, tl, sS, sK);
+    if (g.j == 1) {
+      if (sm) skinny_tile<T, CP, 1, true>(pr, bz, tl, sS, sK);
+      else skinny_tile<T, CP, 1, false>(pr, bz, tl, sS, sK);
+    } else if (O == 4 || g.j == 2) {
+      if (sm) skinny_tile<T, CP, 2, true>(pr, bz, tl, sS, sK);
+      else skinny_tile<T, CP, 2, false>(pr, bz, tl, sS, sK);
+    } else {
+      if (sm) skinny_tile<T, CP, O == 4 ? 2 : 4, true>(pr, bz, tl, sS, sK);
+      else skinny_tile<T, CP, O == 4 ? 2 : 4, false>(pr, bz, tl, sS, sK);
+    }
   }
 }
 
@@ -784,27 +795,16 @@ void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStrea
 #undef MH_SHALLOW
 }
 
-void launch_skinny(bool dbl, bool cplx, int j, const ShallowArgs& a, hipStream_t s) {
+void launch_skinny(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s) {
   if (a.nitems <= 0) return;
   const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, int64_t(1) << 20);
-#define MH_SKINNY(T, CP)                                                                   \
-  switch (j) {                                                                             \
-    case 1: hipLaunchKernelGGL((skinny_kernel<T, CP, 1>), dim3(grid), dim3(KNT), 0, s, a); break; \
-    case 2: hipLaunchKernelGGL((skinny_kernel<T, CP, 2>), dim3(grid), dim3(KNT), 0, s, a); break; \
-    default: hipLaunchKernelGGL((skinny_kernel<T, CP, 4>), dim3(grid), dim3(KNT), 0, s, a); break; \
-  }
   if (dbl) {
-    if (cplx) {
-      // complex FP64 runs 4 outputs per thread: j is 1 or 2
-      if (j == 1) hipLaunchKernelGGL((skinny_kernel<double, true, 1>), dim3(grid), dim3(KNT), 0, s, a);
-      else hipLaunchKernelGGL((skinny_kernel<double, true, 2>), dim3(grid), dim3(KNT), 0, s, a);
-    } else {
-      MH_SKINNY(double, false)
-    }
+    if (cplx) hipLaunchKernelGGL((skinny_kernel<double, true>), dim3(grid), dim3(KNT), 0, s, a);
+    else hipLaunchKernelGGL((skinny_kernel<double, false>), dim3(grid), dim3(KNT), 0, s, a);
   } else {
-    if (cplx) MH_SKINNY(float, true) else MH_SKINNY(float, false)
+    if (cplx) hipLaunchKernelGGL((skinny_kernel<float, true>), dim3(grid), dim3(KNT), 0, s, a);
+    else hipLaunchKernelGGL((skinny_kernel<float, false>), dim3(grid), dim3(KNT), 0, s, a);
   }
-#undef MH_SKINNY
 }
 
 void launch_batch(bool dbl, bool cplx, const BatchArgs& a, const BatchArgs& sp, int wg, size_t lds, hipStream_t s) {

@@ -63,8 +63,12 @@ constexpr int kShallowK = 16;
 // contracted extent is at most kSkinnyMaxK run in the skinny kernel (vector
 // FMAs; one workgroup covers the whole short side and a run of the long side)
 // instead of the MFMA or shallow launches, whose tiles are at least 32 wide.
+// A pair of fewer than kSkinnyMinTiles MFMA tiles stays on the tile routes:
+// dropping its padding gains little, while a skinny item runs all of K in one
+// workgroup and the launch adds to a small batch's critical path.
 constexpr int kSkinnyMaxS = 16;
 constexpr int kSkinnyMaxK = 1024;
+constexpr int64_t kSkinnyMinTiles = 16;
 
 inline bool skinny_pair(int64_t m, int64_t n, int64_t k) {
   return std::min(m, n) <= kSkinnyMaxS && k <= kSkinnyMaxK;
@@ -101,9 +105,8 @@ struct ShallowArgs {
 // Launches the shallow kernel over a.nitems tiles of pairs whose C runs along
 // M (cm) or N (nothing when there are none).
 void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStream_t s);
-// Launches the skinny kernel for pairs of j long positions per thread over
-// a.nitems items (nothing when there are none).
-void launch_skinny(bool dbl, bool cplx, int j, const ShallowArgs& a, hipStream_t s);
+// Launches the skinny kernel over a.nitems items (nothing when there are none).
+void launch_skinny(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s);
 // Launches the persistent kernel (grid = workgroups) over a's items, then the
 // split kernel over split's items (the K slices of split pairs), preceded by a
 // kernel that resets the work counter and the split tiles' arrival counters
@@ -136,7 +139,6 @@ struct Batch {
   std::vector<int2> hitems;
   int32_t qoff[kXcds + 1] = {0};  // the persistent items' per-XCD queues (offsets into hitems)
   int64_t nskinny = 0;   // the skinny items at the end of hitems
-  int64_t nskinny_j[3] = {0, 0, 0};  // of them, those with j = 1, 2, 4 (in that order)
   int64_t nshallow = 0;  // the shallow items before them
   int64_t nshallow_cm[2] = {0, 0};  // of them, those of pairs with C along N, then along M
   int64_t nsplit = 0;    // the slice items before them

@@ -165,7 +165,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     q.k = L.d.K;
     q.batch = L.d.B;
   }
-  std::vector<int2> shallow[2], sliced, skinny[3], xq[kXcds];
+  std::vector<int2> shallow[2], sliced, skinny, xq[kXcds];
   double xload[kXcds] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int sko = skinny_outputs(is_double(B.dtype), cp);
   for (int64_t i : order) {
@@ -184,7 +184,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     const bool split_k = B.allow_split && L.d.K > kShallowK && kt >= kSplitMinKTiles && tiles_of(L) <= kSplitMaxTiles;
     // a pair with an output side of at most kSkinnyMaxS: items of the whole
     // short side by skinny_geom's run of the long side, per batch index
-    const bool sk = !split_k && skinny_pair(L.d.M, L.d.N, L.d.K);
+    const bool sk = !split_k && skinny_pair(L.d.M, L.d.N, L.d.K) && tiles_of(L) >= kSkinnyMinTiles;
     const SkinnyGeom sg = skinny_geom(std::min(L.d.M, L.d.N), sko);
     const int64_t items = sk ? (std::max(L.d.M, L.d.N) + sg.lt - 1) / sg.lt * L.d.B : tiles_of(L);
     if (items >= (int64_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one pair");
@@ -230,7 +230,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
         xload[x] += cost * (double)items;
       }
     } else {
-      std::vector<int2>& dst = sk ? skinny[sg.j == 1 ? 0 : sg.j == 2 ? 1 : 2] : sh ? shallow[L.c_mfast ? 1 : 0] : sliced;
+      std::vector<int2>& dst = sk ? skinny : sh ? shallow[L.c_mfast ? 1 : 0] : sliced;
       for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
     }
     if (!sh && !sk) used[L.tile] = true;
@@ -250,12 +250,8 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     B.nshallow_cm[c] = (int64_t)shallow[c].size();
     B.hitems.insert(B.hitems.end(), shallow[c].begin(), shallow[c].end());
   }
-  B.nskinny = 0;
-  for (int j = 0; j < 3; ++j) {
-    B.nskinny_j[j] = (int64_t)skinny[j].size();
-    B.nskinny += B.nskinny_j[j];
-    B.hitems.insert(B.hitems.end(), skinny[j].begin(), skinny[j].end());
-  }
+  B.nskinny = (int64_t)skinny.size();
+  B.hitems.insert(B.hitems.end(), skinny.begin(), skinny.end());
   const bool dbl = is_double(B.dtype);
   for (int t = 0; t < MUSCLE_HIP_TILE_COUNT; ++t)
     if (used[t]) B.lds = std::max(B.lds, batch_lds_bytes(dbl, cp, t));
@@ -446,17 +442,12 @@ void batch_launch(Batch& b, hipStream_t s) {
       off += b.nshallow_cm[c];
     }
   }
-  // the skinny items, one launch per long positions per thread (1, 2, 4)
-  int64_t off = nmain + b.nsplit + b.nshallow;
-  for (int j = 0; j < 3; ++j) {
-    if (b.nskinny_j[j] > 0) {
-      ShallowArgs a;
-      a.pairs = b.dpairs;
-      a.items = b.ditems + off;
-      a.nitems = b.nskinny_j[j];
-        launch_skinny(is_double(b.dtype), is_complex(b.dtype), 1 << j, a, s);
-    }
-    off += b.nskinny_j[j];
+  if (b.nskinny > 0) {
+    ShallowArgs a;
+    a.pairs = b.dpairs;
+    a.items = b.ditems + nmain + b.nsplit + b.nshallow;
+    a.nitems = b.nskinny;
+    launch_skinny(is_double(b.dtype), is_complex(b.dtype), a, s);
   }
 }
 
@@ -536,7 +527,7 @@ void batch_autotune(Batch& b, int samples, int reps) {
   for (size_t i = 0; i < b.hpairs.size(); ++i) {
     const BatchPair& P = b.hpairs[i];
     if (P.M * (int64_t)P.N * P.NB == 0 || (i < b.solo.size() && b.solo[i])) continue;
-    if (skinny_pair(P.M, P.N, P.K)) continue;  // no tile shape to choose
+    if (b.pinfo[i].route == MUSCLE_HIP_PAIR_SKINNY) continue;  // no tile shape to choose
     by_class[tuning_class(P.M, P.N)].push_back((int64_t)i);
   }
   hipEvent_t e0, e1;

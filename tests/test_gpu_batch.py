@@ -24,6 +24,8 @@ GENERAL = [
     (["i", "j", "s"], (30, 40, 5), ["j", "k"], (40, 35), ["i", "k"]),
     (["i", "j", "i"], (20, 30, 20), ["j", "k"], (30, 40), ["i", "k"]),
     (["a", "b"], (5, 7), ["c"], (9,), ["c", "a", "b"]),
+    (["i", "j"], (3, 40), ["j", "k"], (40, 2500), ["i", "k"]),  # skinny: an output side of 3
+    (["k", "j", "h"], (2200, 300, 2), ["j", "i", "h"], (300, 5, 2), ["h", "k", "i"]),  # skinny, hyper mode
 ]
 
 
@@ -286,7 +288,7 @@ def test_split_k_pairs(T, tol, tile):
         refs.append(bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib))
     batch = ContractionBatch(pairs, tiles=[tile] * len(pairs))
     info = batch.info()
-    assert info["split_items"] > 0 and info["skinny_items"] > 0
+    assert info["split_items"] > 0 and info["shallow_items"] > 0
     batch.run("persistent_nograph")
     check(pairs, refs, tol)
     eager = [c.parent().clone() for c, a, b in pairs]
@@ -327,16 +329,16 @@ def test_batch_contracted_extent_zero():
 
 
 SKINNY = [
-    (["i", "j"], (1, 37), ["j", "k"], (37, 900), ["i", "k"]),  # short side 1, one row of C
-    (["i", "j"], (700, 5), ["j", "k"], (5, 3), ["i", "k"]),  # short side 3 along C's contiguous direction
+    (["i", "j"], (1, 37), ["j", "k"], (37, 2100), ["i", "k"]),  # short side 1, one row of C
+    (["i", "j"], (2100, 5), ["j", "k"], (5, 3), ["i", "k"]),  # short side 3 along C's contiguous direction
     (["j", "i"], (300, 16), ["k", "j"], (2500, 300), ["i", "k"]),  # short side 16, K = 300: three staged chunks
     (["i", "j", "h"], (6, 40, 3), ["j", "k", "h"], (40, 1100, 3), ["h", "k", "i"]),  # hyper modes
     (["i", "j"], (2, 1024), ["j", "k"], (1024, 2000), ["k", "i"]),  # the largest skinny K
     (["i", "j"], (9, 1025), ["j", "k"], (1025, 40), ["i", "k"]),  # K = 1025: the MFMA route (K slices)
     (["i"], (4000,), ["k"], (4,), ["k", "i"]),  # outer product (K = 1)
-    (["a", "b"], (5, 7), ["b", "c"], (7, 5), ["a", "c"]),  # both sides short
-    (["i", "j", "s"], (12, 30, 3), ["j", "k"], (30, 513), ["i", "k"]),  # index summed in A alone
-    (["i", "j", "i"], (10, 30, 10), ["j", "k"], (30, 700), ["i", "k"]),  # diagonal of A
+    (["a", "b"], (5, 7), ["b", "c"], (7, 5), ["a", "c"]),  # one tile: stays on the shallow route
+    (["i", "j", "s"], (12, 30, 3), ["j", "k"], (30, 2100), ["i", "k"]),  # index summed in A alone
+    (["i", "j", "i"], (10, 30, 10), ["j", "k"], (30, 2100), ["i", "k"]),  # diagonal of A
     (["i", "j"], (15, 9), ["j", "k"], (9, 5000), ["i", "k"]),  # ragged last run of the long side
 ]
 
@@ -344,12 +346,12 @@ SKINNY = [
 @pytest.mark.parametrize("T, tol", [(np.complex128, 1e-12), (np.float64, 1e-12), (np.complex64, 2e-5),
                                     (np.float32, 2e-5)])
 def test_skinny_pairs(T, tol):
-    """Pairs with an output side of at most 16 and K <= 1024 run in the skinny
-    vector-FMA kernel (whole short side per workgroup, long-side operand
-    streamed into registers): short sides 1-16 on either side of C, K from 1 to
-    1024 over several staged chunks, hyper, summed and diagonal modes match the
-    oracle; graph replays equal the eager launch bit for bit, and a pair's
-    result is the same alone as in the batch."""
+    """Pairs with an output side of at most 16, K <= 1024 and at least 16 MFMA
+    tiles run in the skinny vector-FMA kernel (whole short side per workgroup,
+    long-side operand streamed into registers): short sides 1-16 on either side
+    of C, K from 1 to 1024 over several staged chunks, hyper, summed and
+    diagonal modes match the oracle; graph replays equal the eager launch bit
+    for bit, and a pair's result is the same alone as in the batch."""
     rng = np.random.default_rng(21)
     pairs, refs = [], []
     for ia, sa, ib, sb, ic in SKINNY:
@@ -361,7 +363,7 @@ def test_skinny_pairs(T, tol):
         refs.append(bb.binary_einsum_general(ic, A.astype(np.complex128), ia, B.astype(np.complex128), ib))
     batch = ContractionBatch(pairs)
     routes = [batch.pair_info(i)["route"] for i in range(len(pairs))]
-    assert routes == ["skinny"] * 5 + ["split_k"] + ["skinny"] * 5, routes
+    assert routes == ["skinny"] * 5 + ["split_k", "skinny", "shallow"] + ["skinny"] * 3, routes
     assert batch.info()["skinny_items"] > 0
     batch.run("persistent_nograph")
     check(pairs, refs, tol)
