@@ -72,7 +72,8 @@ __device__ __forceinline__ void split_tile_fixup(const BatchPair& pr, int bz, in
       acc.re += v.re;
       acc.im += v.im;
     }
-    store_elem_nt<T, CP>(pr.C, offC + pr.rowC[m0 + m] + pr.colC[n0 + n], acc);
+    if (pr.c_stream) store_elem_nt<T, CP>(pr.C, offC + pr.rowC[m0 + m] + pr.colC[n0 + n], acc);
+    else store_elem<T, CP>(pr.C, offC + pr.rowC[m0 + m] + pr.colC[n0 + n], acc);
   }
 }
 
@@ -271,7 +272,7 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
           re = acc0[i][j][r];
           im = acc1[i][j][r];
         }
-        if constexpr (split) store_elem<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});  // read back soon
+        if (split || !pr.c_stream) store_elem<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});  // partials: read back soon
         else store_elem_nt<T, CP>(dst, cn + crow[i][r], Cplx<T>{re, im});
       }
     }
@@ -400,7 +401,7 @@ struct ShallowItem {
   const void* A;
   const void* B;
   void* C;
-  int BM, BN, mr, nr, K, a_kmajor, b_kmajor, c_mfast;
+  int BM, BN, mr, nr, K, a_kmajor, b_kmajor, c_mfast, c_stream;
 };
 
 // Table phase of item `it`: one lookup per thread (rows, columns, K offsets,
@@ -426,6 +427,7 @@ __device__ __forceinline__ ShallowItem shallow_tables(const ShallowArgs& a, int6
   s.a_kmajor = pr.a_kmajor;
   s.b_kmajor = pr.b_kmajor;
   s.c_mfast = pr.c_mfast;
+  s.c_stream = pr.c_stream;
   const int e = tid, BM = s.BM, BN = s.BN, K = s.K;
   if (e < BM) {
     soff[SOFF_ROWA + e] = e < s.mr ? pr.rowA[m0 + e] : 0;
@@ -520,7 +522,10 @@ __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_
 #pragma unroll
   for (int i = 0; i < SOUT; ++i) {
     const int m = CM ? f : v0 + vs * i, n = CM ? v0 + vs * i : f;
-    if (m < s.mr && n < s.nr) store_elem_nt<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
+    if (m < s.mr && n < s.nr) {
+      if (s.c_stream) store_elem_nt<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
+      else store_elem<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
+    }
   }
 }
 
@@ -703,13 +708,14 @@ __device__ __forceinline__ void skinny_tile(const BatchPair& pr, int bz, int tl,
     if (l >= L) continue;
     const int64_t cl = offC + cL[l];
 #pragma unroll
-    for (int i = 0; i < QM; ++i)
-      if (q0 + i < S) store_elem_nt<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
+    for (int i = 0; i < QM; ++i) {
+      if (q0 + i >= S) continue;
+      if (pr.c_stream) store_elem_nt<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
+      else store_elem<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
+    }
   }
 }
 
-// One launch per J (the items of pairs with the same long positions per
-// thread), so each has the registers its own ring and accumulators need.
 // One launch for every skinny pair: each item takes the variant of its pair's
 // long positions per thread (J = 1, 2 or 4; complex FP64 uses 1 and 2).
 template <typename T, bool CP>

@@ -21,6 +21,7 @@ struct BatchPair {
   int32_t M, N, K, NB;  // fused extents and batch (hyper) count
   int32_t tiles_m, tiles_n, tile, a_kmajor, b_kmajor;
   int32_t c_mfast;  // C's contiguous direction is M (else N)
+  int32_t c_stream; // C written with nontemporal stores (kStreamStoreBytes)
   // split K: ksplit slices of kchunk elements per output tile (1 and K when
   // not split); the slices' partial tiles and the tiles' arrival counters
   int32_t ksplit, kchunk;

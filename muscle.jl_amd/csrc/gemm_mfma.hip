@@ -289,7 +289,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_tile_kernel(GemmArgs p) {
           im = acc1[i][j][r];
         }
         if (to_partial) store_elem<T, CC>(p.partial, cn + (int64_t)m * p.N, Cplx<T>{re, im});
-        else store_elem_nt<S, CC>(p.C, cn + crow[i][r], Cplx<S>{(S)re, (S)im});
+        else if (p.c_stream) store_elem_nt<S, CC>(p.C, cn + crow[i][r], Cplx<S>{(S)re, (S)im});
+        else store_elem<S, CC>(p.C, cn + crow[i][r], Cplx<S>{(S)re, (S)im});
       }
     }
 }

@@ -68,7 +68,13 @@ struct GemmArgs {
   // compute type and a second kernel adds the slices in order into C.
   int32_t ksplit, kchunk;
   void* partial;
+  int32_t c_stream;  // store C with nontemporal (streaming) stores
 };
+// C outputs of at least this many bytes are written with nontemporal stores:
+// they do not stay in the caches anyway (writing C bounds small-K products,
+// which this speeds up), while smaller outputs are often read again soon (the
+// next contraction of a tree).
+constexpr double kStreamStoreBytes = double(int64_t(16) << 20);
 // dbl: FP64 storage (else FP32); fp32_math: FP64 storage with FP32 MFMA
 // arithmetic (the FP32 rung); ca/cb: operand is complex.
 void launch_gemm_mfma(bool dbl, bool fp32_math, bool ca, bool cb, const GemmArgs& g, hipStream_t s);
