@@ -295,13 +295,14 @@ __global__ __launch_bounds__(NT, kBatchWgPerCu) void batch_kernel(BatchArgs a) {
   // touches (see batch_item_offset)
   int* s_item = reinterpret_cast<int*>(smem_raw + a.item_off);
   // this workgroup's XCD queue first; once it is drained, the others in turn
-  const int x0 = blockIdx.x % kXcds;
+  // (a small launch has one queue)
+  const int x0 = a.nq > 1 ? blockIdx.x % kXcds : 0;
   int q = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       int item = -1;
-      for (; q < kXcds; ++q) {
-        const int x = (x0 + q) % kXcds;
+      for (; q < a.nq; ++q) {
+        const int x = (x0 + q) % a.nq;
         const int i = a.qoff[x] + atomicAdd(&a.counter[x], 1);
         if (i < a.qoff[x + 1]) {
           item = i;

@@ -42,12 +42,16 @@ constexpr TileShape kTileShapes[MUSCLE_HIP_TILE_COUNT] = {{64, 64, 2, 2}, {32, 1
 // on XCD b % kXcds): each pair's tiles are cut into kXcds contiguous runs, one
 // per queue, so the tiles that share A and B panels meet in one L2.
 constexpr int kXcds = 8;
+// per-XCD queues are used when the persistent items number at least this
+// many per workgroup of the launch (else one queue)
+constexpr int kXcdQueueItems = 4;
 
 struct BatchArgs {
   const BatchPair* pairs;
   const int2* items;  // (pair, tile within pair), per queue largest pairs first
   int32_t* counter;   // work queue heads (kXcds of them)
   int32_t qoff[kXcds + 1];  // queue q holds items [qoff[q], qoff[q + 1])
+  int32_t nq;               // queues in use: kXcds, or 1 (queue 0)
   int32_t* tilecnt;   // arrival counters of the split pairs' tiles, cleared with the queue head
   int64_t ntilecnt;
   int64_t nitems;
@@ -139,6 +143,7 @@ struct Batch {
   // then the shallow items, then the skinny items
   std::vector<int2> hitems;
   int32_t qoff[kXcds + 1] = {0};  // the persistent items' per-XCD queues (offsets into hitems)
+  int32_t nq = 1;                 // queues in use (kXcds or 1)
   int64_t nskinny = 0;   // the skinny items at the end of hitems
   int64_t nshallow = 0;  // the shallow items before them
   int64_t nshallow_cm[2] = {0, 0};  // of them, those of pairs with C along N, then along M

@@ -165,7 +165,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     q.k = L.d.K;
     q.batch = L.d.B;
   }
-  std::vector<int2> shallow[2], sliced, skinny, xq[kXcds];
+  std::vector<int2> shallow[2], sliced, skinny, xq[kXcds], single;
   double xload[kXcds] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int sko = skinny_outputs(is_double(B.dtype), cp);
   for (int64_t i : order) {
@@ -216,7 +216,9 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     }
     if (!sk && !sh && ks == 1) {
       // persistent tiles: kXcds contiguous runs, one per XCD queue; a pair of
-      // fewer tiles goes whole to the queue with the least work so far
+      // fewer tiles goes whole to the queue with the least work so far (and
+      // all of them, in order, to the single queue of a small launch)
+      for (int64_t t = 0; t < items; ++t) single.push_back(make_int2((int)i, (int)t));
       const double cost = (double)L.cost;
       if (items >= kXcds) {
         for (int x = 0; x < kXcds; ++x) {
@@ -238,10 +240,18 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
   size_t nq = 0;
   for (int x = 0; x < kXcds; ++x) nq += xq[x].size();
   if (nq >= (size_t(1) << 31)) fail(MUSCLE_HIP_UNSUPPORTED, "too many tiles in one batch");
+  // per-XCD queues pay off when every workgroup has several items of its own;
+  // in a small launch the workgroups would mostly scan empty queues
+  B.nq = (int64_t)nq >= int64_t(kXcdQueueItems) * B.workgroups ? kXcds : 1;
   B.qoff[0] = 0;
-  for (int x = 0; x < kXcds; ++x) {
-    B.hitems.insert(B.hitems.end(), xq[x].begin(), xq[x].end());
-    B.qoff[x + 1] = (int32_t)B.hitems.size();
+  if (B.nq == 1) {
+    B.hitems.insert(B.hitems.end(), single.begin(), single.end());
+    for (int x = 0; x < kXcds; ++x) B.qoff[x + 1] = (int32_t)B.hitems.size();
+  } else {
+    for (int x = 0; x < kXcds; ++x) {
+      B.hitems.insert(B.hitems.end(), xq[x].begin(), xq[x].end());
+      B.qoff[x + 1] = (int32_t)B.hitems.size();
+    }
   }
   B.nsplit = (int64_t)sliced.size();
   B.nshallow = (int64_t)(shallow[0].size() + shallow[1].size());
@@ -418,6 +428,7 @@ void batch_launch(Batch& b, hipStream_t s) {
     a.items = b.ditems;
     a.counter = b.dcounter;
     for (int x = 0; x <= kXcds; ++x) a.qoff[x] = b.qoff[x];
+    a.nq = b.nq;
     a.tilecnt = b.dtilecnt;
     a.ntilecnt = b.ntilecnt;
     a.nitems = nmain;
