@@ -529,7 +529,7 @@ void run_contraction(Handle& h, Plan& p, const muscle_hip_tensor_t& c, const mus
     g.ksplit = q.gemm_ksplit;
     g.kchunk = q.gemm_kchunk;
     g.partial = base + w.partial_off;
-    g.c_stream = (double)d.M * (double)d.N * (double)d.B * (double)csize >= kStreamStoreBytes;
+    g.c_stream = csize >= 16 && (double)d.M * (double)d.N * (double)d.B * (double)csize >= kStreamStoreBytes;
     // grid.y carries the batch; split launches beyond the 65535 limit
     for (int64_t b0 = 0; b0 < d.B; b0 += 65535) {
       g.batch = (int32_t)std::min<int64_t>(65535, d.B - b0);

@@ -358,7 +358,7 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
     P.a_kmajor = L.a_kmajor;
     P.b_kmajor = L.b_kmajor;
     P.c_mfast = L.c_mfast;
-    P.c_stream = (double)L.d.M * (double)L.d.N * (double)L.d.B * (double)dtype_size(L.d.dt[2]) >= kStreamStoreBytes;
+    P.c_stream = dtype_size(L.d.dt[2]) >= 16 && (double)L.d.M * (double)L.d.N * (double)L.d.B * (double)dtype_size(L.d.dt[2]) >= kStreamStoreBytes;
     P.ksplit = B.ksplit.empty() ? 1 : B.ksplit[(size_t)i];
     P.kchunk = B.kchunk.empty() || B.kchunk[(size_t)i] == 0 ? P.K : B.kchunk[(size_t)i];
     P.partial = base + l.partial + (B.partial_off.empty() ? 0 : B.partial_off[(size_t)i]);

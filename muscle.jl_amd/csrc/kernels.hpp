@@ -70,10 +70,12 @@ struct GemmArgs {
   void* partial;
   int32_t c_stream;  // store C with nontemporal (streaming) stores
 };
-// C outputs of at least this many bytes are written with nontemporal stores:
-// they do not stay in the caches anyway (writing C bounds small-K products,
-// which this speeds up), while smaller outputs are often read again soon (the
-// next contraction of a tree).
+// Complex FP64 outputs of at least this many bytes are written with
+// nontemporal stores: they do not stay in the caches anyway (writing C bounds
+// small-K products, which this speeds up), while smaller outputs are often
+// read again soon (the next contraction of a tree).  Elements of 8 bytes or
+// less keep ordinary stores: written nontemporally, the L2 no longer merges
+// their partial lines (config 4 in ComplexF32: 138.5 against 149.6 ms).
 constexpr double kStreamStoreBytes = double(int64_t(16) << 20);
 // dbl: FP64 storage (else FP32); fp32_math: FP64 storage with FP32 MFMA
 // arithmetic (the FP32 rung); ca/cb: operand is complex.
