@@ -711,7 +711,9 @@ __device__ __forceinline__ void skinny_tile(const BatchPair& pr, int bz, int tl,
 #pragma unroll
     for (int i = 0; i < QM; ++i) {
       if (q0 + i >= S) continue;
-      if (pr.c_stream) store_elem_nt<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
+      // streaming stores only with lanes along C's contiguous direction (SM):
+      // otherwise a lane's few consecutive elements leave partial lines
+      if (SM && pr.c_stream) store_elem_nt<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
       else store_elem<T, CP>(pr.C, cl + sK[KOFF_C + q0 + i], acc[j][i]);
     }
   }
