@@ -292,7 +292,7 @@ typedef struct {
   int64_t table_bytes;/* device bytes of offset tables and pair descriptors */
   int64_t shallow_items;/* of `items`, the tiles of pairs with K <= 16 (vector-FMA kernel) */
   int64_t split_items;/* of `items`, K slices of pairs with <= 2 output tiles and K >= 512 */
-  int64_t skinny_items;/* of `items`, the items of pairs with min(m, n) <= 16 and K <= 1024 (skinny vector-FMA kernel) */
+  int64_t skinny_items;/* of `items`, the items of pairs with min(m, n) <= 16, K <= 1024 and 16+ MFMA tiles (skinny vector-FMA kernel) */
 } muscle_hip_batch_info_t;
 
 /* Plans `n` independent binary contractions C_i = A_i * B_i, each with the
@@ -327,7 +327,7 @@ typedef enum {
   MUSCLE_HIP_PAIR_SPLIT_K = 2, /* few output tiles and a long K: K slices, summed in slice order */
   MUSCLE_HIP_PAIR_SHALLOW = 3, /* K <= 16: the vector-FMA launch */
   MUSCLE_HIP_PAIR_SOLO = 4,    /* 2^29 or more index points: a single-contraction launch of its own */
-  MUSCLE_HIP_PAIR_SKINNY = 5   /* min(m, n) <= 16 and K <= 1024: the skinny vector-FMA launch (whole short side per item) */
+  MUSCLE_HIP_PAIR_SKINNY = 5   /* min(m, n) <= 16, K <= 1024, 16+ MFMA tiles, not split: the skinny vector-FMA launch (whole short side per item) */
 } muscle_hip_pair_route_t;
 
 typedef struct {
