@@ -203,6 +203,7 @@ struct PlannedPair {
   const void* b = nullptr;
   void* c = nullptr;
   const int64_t* dtab = nullptr;  // device tables of this pair; null: the batch's own table region
+  const std::vector<int64_t>* htab = nullptr;  // the host copy its TabRefs index
   bool solo = false;              // runs as a single-contraction launch, not in the work list
 };
 

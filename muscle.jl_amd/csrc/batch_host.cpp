@@ -9,6 +9,7 @@
 #include <fstream>
 #include <mutex>
 #include <numeric>
+#include <utility>
 
 #include "lowering.hpp"
 
@@ -54,6 +55,12 @@ TabRef add_table(std::vector<int64_t>& h, const std::vector<Dim>& dims, int slot
   r.P = (uint32_t)P;
   r.single = hi.empty() ? 1u : 0u;
   return r;
+}
+
+// Host value of entry i of a table (the device OffTab's lookup).
+int64_t tab_at(const std::vector<int64_t>& h, const TabRef& r, int64_t i) {
+  if (r.single) return h[r.lo + (size_t)i];
+  return h[r.lo + (size_t)(i % r.P)] + h[r.hi + (size_t)(i / r.P)];
 }
 
 OffTab make_tab(const int64_t* base, const TabRef& r) {
@@ -105,6 +112,7 @@ void plan_pair(const Handle& h, PlannedPair& L, std::vector<int64_t>& htab, int 
     fail(MUSCLE_HIP_UNSUPPORTED, "batched pair with an index class of 2^31 or more elements");
   const std::vector<Dim>* cls[4] = {&d.m, &d.n, &d.k, &d.batch};
   for (int t = 0; t < 9; ++t) L.t[t] = add_table(htab, *cls[T3[t][0]], T3[t][1], single_max);
+  L.htab = &htab;
   if (tile < 0 || tile >= MUSCLE_HIP_TILE_COUNT) {
     const int tc = h.tune[tuning_class(d.M, d.N)];
     tile = tc >= 0 ? tc : default_tile(d.M, d.N);
@@ -231,9 +239,37 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
         for (int64_t t = 0; t < items; ++t) xq[x].push_back(make_int2((int)i, (int)t));
         xload[x] += cost * (double)items;
       }
+    } else if (sh || sk) {
+      // the vector-FMA kernels are bound by writing C: their items go in the
+      // order of their corners' addresses in C, so the tiles that run at the
+      // same time write neighbouring memory (with C's modes in a random order,
+      // the tile order along M and N would scatter them over C)
+      std::vector<int2>& dst = sk ? skinny : shallow[L.c_mfast ? 1 : 0];
+      const size_t first = dst.size();
+      for (int64_t t = 0; t < items; ++t) dst.push_back(make_int2((int)i, (int)t));
+      if (L.htab && items > 1) {
+        const std::vector<int64_t>& h = *L.htab;
+        std::vector<std::pair<int64_t, int32_t>> key((size_t)items);
+        const int64_t tiles_mn = sk ? (std::max(L.d.M, L.d.N) + sg.lt - 1) / sg.lt
+                                    : ((L.d.M + ts.bm - 1) / ts.bm) * ((L.d.N + ts.bn - 1) / ts.bn);
+        const int64_t tiles_n = (L.d.N + ts.bn - 1) / ts.bn;
+        for (int64_t t = 0; t < items; ++t) {
+          const int64_t bz = t / tiles_mn, r = t - bz * tiles_mn;
+          int64_t off = tab_at(h, L.t[8], bz);
+          if (sk) {
+            const int64_t l = r * sg.lt;
+            off += L.d.M <= L.d.N ? tab_at(h, L.t[5], l) : tab_at(h, L.t[4], l);
+          } else {
+            const int64_t tm = r / tiles_n, tn = r - tm * tiles_n;
+            off += tab_at(h, L.t[4], tm * ts.bm) + tab_at(h, L.t[5], tn * ts.bn);
+          }
+          key[(size_t)t] = std::make_pair(off < 0 ? -off : off, (int32_t)t);
+        }
+        std::stable_sort(key.begin(), key.end());
+        for (int64_t t = 0; t < items; ++t) dst[first + (size_t)t].y = key[(size_t)t].second;
+      }
     } else {
-      std::vector<int2>& dst = sk ? skinny : sh ? shallow[L.c_mfast ? 1 : 0] : sliced;
-      for (int64_t t = 0; t < items * ks; ++t) dst.push_back(make_int2((int)i, (int)t));
+      for (int64_t t = 0; t < items * ks; ++t) sliced.push_back(make_int2((int)i, (int)t));
     }
     if (!sh && !sk) used[L.tile] = true;
   }
