@@ -382,6 +382,11 @@ __global__ __launch_bounds__(NT, kBatchWgPerCu) void split_kernel(BatchArgs a) {
 constexpr int SKC = 8;                // K elements staged per pass
 constexpr int SMAXE = 160;            // BM + BN of every tile shape
 constexpr int SNT = 512;              // threads per workgroup
+// Workgroups of a shallow launch (a grid-stride loop over the items).  A
+// launch of up to 2^20 items gives each workgroup one; resident grids of 512
+// to 2048 workgroups that loop over items measured no faster for complex
+// FP64 and 2-6 % slower for complex FP32.
+constexpr int64_t kShallowGrid = int64_t(1) << 20;
 constexpr int SOUT = 4096 / SNT;      // outputs per thread
 constexpr int SGA = (128 * SKC + SNT - 1) / SNT;  // gathered elements per operand and thread
 constexpr int SOFF_ROWA = 0, SOFF_ROWC = 128, SOFF_COLB = 256, SOFF_COLC = 384, SOFF_KA = 512,
@@ -402,7 +407,7 @@ struct ShallowItem {
   const void* A;
   const void* B;
   void* C;
-  int BM, BN, mr, nr, K, a_kmajor, b_kmajor, c_mfast, c_stream;
+  int BM, BN, mr, nr, K, a_kmajor, b_kmajor, c_mfast;
 };
 
 // Table phase of item `it`: one lookup per thread (rows, columns, K offsets,
@@ -428,7 +433,6 @@ __device__ __forceinline__ ShallowItem shallow_tables(const ShallowArgs& a, int6
   s.a_kmajor = pr.a_kmajor;
   s.b_kmajor = pr.b_kmajor;
   s.c_mfast = pr.c_mfast;
-  s.c_stream = pr.c_stream;
   const int e = tid, BM = s.BM, BN = s.BN, K = s.K;
   if (e < BM) {
     soff[SOFF_ROWA + e] = e < s.mr ? pr.rowA[m0 + e] : 0;
@@ -514,7 +518,12 @@ __device__ __forceinline__ void shallow_compute(const ShallowItem& s, const Cplx
   }
 }
 
-template <typename T, bool CP, bool CM>
+// Outputs of a partial tile that lie outside C go here, so that every thread
+// issues the same number of stores whatever the tile: the wait for the next
+// item's gathered operands can then leave this item's stores in flight.
+__device__ double2 shallow_sink[SNT];
+
+template <typename T, bool CP, bool CM, bool CS>
 __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_t* soff, const Cplx<T>* acc) {
   const int tid = threadIdx.x;
   const int LF = CM ? s.BM : s.BN;
@@ -523,22 +532,24 @@ __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_
 #pragma unroll
   for (int i = 0; i < SOUT; ++i) {
     const int m = CM ? f : v0 + vs * i, n = CM ? v0 + vs * i : f;
-    if (m < s.mr && n < s.nr) {
-      if (s.c_stream) store_elem_nt<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
-      else store_elem<T, CP>(s.C, offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n], acc[i]);
-    }
+    const bool in = m < s.mr && n < s.nr;
+    void* const base = in ? s.C : (void*)shallow_sink;
+    const int64_t o = in ? offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n] : (int64_t)tid;
+    if constexpr (CS) store_elem_nt<T, CP>(base, o, acc[i]);
+    else store_elem<T, CP>(base, o, acc[i]);
   }
 }
 
-// Two 8-wave workgroups per CU (about 126 registers for complex FP64: the
+// Two 8-wave workgroups per CU (about 122 registers for complex FP64: the
 // next item's gathered pass is held across the stores); one launch per C
-// orientation (CM: C runs along M).  Each workgroup loops over items; the
-// next item's table phase and first gather are issued before the current
-// item's stores, so its loads are not queued behind them (vector memory
-// operations complete in order) and the stores drain while the next item is
-// computed.  soff is double-buffered (the stores read the current item's C
-// offsets after the next item's table phase).
-template <typename T, bool CP, bool CM>
+// orientation (CM: C runs along M) and store kind (CS: nontemporal).  A
+// workgroup that loops over items issues the next item's table phase and
+// first gather before the current item's stores, so its loads are not queued
+// behind them (vector memory operations complete in order), and stashes the
+// gathered pass right after the stores with a wait that leaves the SOUT
+// stores in flight.  soff is double-buffered (the stores read the current
+// item's C offsets after the next item's table phase).
+template <typename T, bool CP, bool CM, bool CS>
 __global__ __launch_bounds__(SNT) void shallow_kernel(
     ShallowArgs a) {
   __shared__ Cplx<T> sab[SMAXE * SKC];
@@ -550,30 +561,35 @@ __global__ __launch_bounds__(SNT) void shallow_kernel(
   __syncthreads();
   Cplx<T> ra[SGA], rb[SGA];
   shallow_gather<T, CP>(cur, soff[0], 0, ra, rb);
+  shallow_stash<T>(cur, sab, ra, rb);
   for (;;) {
     Cplx<T> acc[SOUT];
 #pragma unroll
     for (int i = 0; i < SOUT; ++i) acc[i] = Cplx<T>{T(0), T(0)};
-    for (int k0 = 0;;) {
-      __syncthreads();  // the readers of sab (and of soff[b ^ 1], by the previous item) are done
+    __syncthreads();  // the item's first pass is in sab
+    shallow_compute<T, CP, CM>(cur, sab, min(SKC, cur.K), acc);
+    for (int k0 = SKC; k0 < cur.K; k0 += SKC) {
+      shallow_gather<T, CP>(cur, soff[b], k0, ra, rb);
+      __syncthreads();  // the readers of sab are done
       shallow_stash<T>(cur, sab, ra, rb);
       __syncthreads();
       shallow_compute<T, CP, CM>(cur, sab, min(SKC, cur.K - k0), acc);
-      k0 += SKC;
-      if (k0 >= cur.K) break;
-      __syncthreads();
-      shallow_gather<T, CP>(cur, soff[b], k0, ra, rb);
     }
     const int64_t it2 = it + gridDim.x;
     const bool more = it2 < a.nitems;
     ShallowItem nxt;
     if (more) {
+      // soff[b ^ 1] was last read by the previous item's stores, before the
+      // barrier at the top of this item
       nxt = shallow_tables(a, it2, soff[b ^ 1]);
-      __syncthreads();
+      __syncthreads();  // also: the readers of sab are done
       shallow_gather<T, CP>(nxt, soff[b ^ 1], 0, ra, rb);
     }
-    shallow_store<T, CP, CM>(cur, soff[b], acc);
+    shallow_store<T, CP, CM, CS>(cur, soff[b], acc);
     if (!more) break;
+    // the stash waits for the gathered operands only: they were issued
+    // before the stores, and every thread issues SOUT stores
+    shallow_stash<T>(nxt, sab, ra, rb);
     cur = nxt;
     b ^= 1;
     it = it2;
@@ -781,13 +797,18 @@ size_t batch_lds_bytes(bool dbl, bool cplx, int tile) {
   return bufs * pl * BK * ((size_t)(t.bm + LDP) + (t.bn + LDP)) * el;
 }
 
-void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStream_t s) {
+void launch_shallow(bool dbl, bool cplx, bool cm, bool cs, const ShallowArgs& a, hipStream_t s) {
   if (a.nitems <= 0) return;
-  // a grid-stride loop over the items; 2^20 workgroups fill the GPU many times
-  const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, int64_t(1) << 20);
-#define MH_SHALLOW(T, CP)                                                                           \
-  if (cm) hipLaunchKernelGGL((shallow_kernel<T, CP, true>), dim3(grid), dim3(SNT), 0, s, a);       \
-  else hipLaunchKernelGGL((shallow_kernel<T, CP, false>), dim3(grid), dim3(SNT), 0, s, a);
+  const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, kShallowGrid);
+#define MH_SHALLOW_CS(T, CP, CM)                                                                    \
+  if (cs) hipLaunchKernelGGL((shallow_kernel<T, CP, CM, true>), dim3(grid), dim3(SNT), 0, s, a);   \
+  else hipLaunchKernelGGL((shallow_kernel<T, CP, CM, false>), dim3(grid), dim3(SNT), 0, s, a);
+#define MH_SHALLOW(T, CP)       \
+  if (cm) {                     \
+    MH_SHALLOW_CS(T, CP, true)  \
+  } else {                      \
+    MH_SHALLOW_CS(T, CP, false) \
+  }
   if (dbl) {
     if (cplx) {
       MH_SHALLOW(double, true)
@@ -802,6 +823,7 @@ void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStrea
     }
   }
 #undef MH_SHALLOW
+#undef MH_SHALLOW_CS
 }
 
 void launch_skinny(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s) {

@@ -109,7 +109,7 @@ struct ShallowArgs {
 
 // Launches the shallow kernel over a.nitems tiles of pairs whose C runs along
 // M (cm) or N (nothing when there are none).
-void launch_shallow(bool dbl, bool cplx, bool cm, const ShallowArgs& a, hipStream_t s);
+void launch_shallow(bool dbl, bool cplx, bool cm, bool cs, const ShallowArgs& a, hipStream_t s);
 // Launches the skinny kernel over a.nitems items (nothing when there are none).
 void launch_skinny(bool dbl, bool cplx, const ShallowArgs& a, hipStream_t s);
 // Launches the persistent kernel (grid = workgroups) over a's items, then the
@@ -146,7 +146,9 @@ struct Batch {
   int32_t nq = 1;                 // queues in use (kXcds or 1)
   int64_t nskinny = 0;   // the skinny items at the end of hitems
   int64_t nshallow = 0;  // the shallow items before them
-  int64_t nshallow_cm[2] = {0, 0};  // of them, those of pairs with C along N, then along M
+  // of them, those of pairs with C along N, along M, then the same two with
+  // streaming stores for C
+  int64_t nshallow_cm[4] = {0, 0, 0, 0};
   int64_t nsplit = 0;    // the slice items before them
   // split K (per pair of the work list, by position): slices, K per slice,
   // offsets of the pair's partial tiles (bytes) and arrival counters

@@ -57,6 +57,13 @@ TabRef add_table(std::vector<int64_t>& h, const std::vector<Dim>& dims, int slot
   return r;
 }
 
+// C is written with nontemporal stores: complex FP64 outputs of
+// kStreamStoreBytes or more.
+bool c_stream_of(const ContractionDesc& d) {
+  const double es = (double)dtype_size(d.dt[2]);
+  return es >= 16 && (double)d.M * (double)d.N * (double)d.B * es >= kStreamStoreBytes;
+}
+
 // Host value of entry i of a table (the device OffTab's lookup).
 int64_t tab_at(const std::vector<int64_t>& h, const TabRef& r, int64_t i) {
   if (r.single) return h[r.lo + (size_t)i];
@@ -173,7 +180,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     q.k = L.d.K;
     q.batch = L.d.B;
   }
-  std::vector<int2> shallow[2], sliced, skinny, xq[kXcds], single;
+  std::vector<int2> shallow[4], sliced, skinny, xq[kXcds], single;
   double xload[kXcds] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int sko = skinny_outputs(is_double(B.dtype), cp);
   for (int64_t i : order) {
@@ -244,7 +251,7 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
       // order of their corners' addresses in C, so the tiles that run at the
       // same time write neighbouring memory (with C's modes in a random order,
       // the tile order along M and N would scatter them over C)
-      std::vector<int2>& dst = sk ? skinny : shallow[L.c_mfast ? 1 : 0];
+      std::vector<int2>& dst = sk ? skinny : shallow[(L.c_mfast ? 1 : 0) + (c_stream_of(L.d) ? 2 : 0)];
       const size_t first = dst.size();
       for (int64_t t = 0; t < items; ++t) dst.push_back(make_int2((int)i, (int)t));
       if (L.htab && items > 1) {
@@ -290,9 +297,9 @@ void finish_batch(Handle& h, Batch& B, const std::vector<const PlannedPair*>& lw
     }
   }
   B.nsplit = (int64_t)sliced.size();
-  B.nshallow = (int64_t)(shallow[0].size() + shallow[1].size());
+  B.nshallow = (int64_t)(shallow[0].size() + shallow[1].size() + shallow[2].size() + shallow[3].size());
   B.hitems.insert(B.hitems.end(), sliced.begin(), sliced.end());
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 4; ++c) {
     B.nshallow_cm[c] = (int64_t)shallow[c].size();
     B.hitems.insert(B.hitems.end(), shallow[c].begin(), shallow[c].end());
   }
@@ -394,7 +401,7 @@ void place_batch(Batch& B, const std::vector<const PlannedPair*>& lw, char* base
     P.a_kmajor = L.a_kmajor;
     P.b_kmajor = L.b_kmajor;
     P.c_mfast = L.c_mfast;
-    P.c_stream = dtype_size(L.d.dt[2]) >= 16 && (double)L.d.M * (double)L.d.N * (double)L.d.B * (double)dtype_size(L.d.dt[2]) >= kStreamStoreBytes;
+    P.c_stream = c_stream_of(L.d);
     P.ksplit = B.ksplit.empty() ? 1 : B.ksplit[(size_t)i];
     P.kchunk = B.kchunk.empty() || B.kchunk[(size_t)i] == 0 ? P.K : B.kchunk[(size_t)i];
     P.partial = base + l.partial + (B.partial_off.empty() ? 0 : B.partial_off[(size_t)i]);
@@ -477,15 +484,15 @@ void batch_launch(Batch& b, hipStream_t s) {
     launch_batch(is_double(b.dtype), is_complex(b.dtype), a, sp, b.workgroups, lds, s);
   }
   if (b.nshallow > 0) {
-    // one launch per C orientation
+    // one launch per C orientation and store kind
     int64_t off = nmain + b.nsplit;
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 4; ++c) {
       if (b.nshallow_cm[c] > 0) {
         ShallowArgs a;
         a.pairs = b.dpairs;
         a.items = b.ditems + off;
         a.nitems = b.nshallow_cm[c];
-        launch_shallow(is_double(b.dtype), is_complex(b.dtype), c == 1, a, s);
+        launch_shallow(is_double(b.dtype), is_complex(b.dtype), (c & 1) != 0, c >= 2, a, s);
       }
       off += b.nshallow_cm[c];
     }

@@ -31,13 +31,31 @@ struct Elem<float, true> {
   typedef float2 type;
 };
 
+// Every operand, output and partial buffer is device (global) memory.
+// Addressed as such, its loads and stores are global rather than flat
+// instructions: a flat access also counts against the LDS wait counter, so
+// each wait for an LDS read or write would drain the outstanding stores of C.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MH_GLOBAL __attribute__((address_space(1)))
+#else
+#define MH_GLOBAL
+#endif
+template <typename E>
+__device__ inline const MH_GLOBAL E* gptr(const void* p) {
+  return (const MH_GLOBAL E*)p;
+}
+template <typename E>
+__device__ inline MH_GLOBAL E* gptr(void* p) {
+  return (MH_GLOBAL E*)p;
+}
+
 template <typename T, bool C>
 __device__ inline Cplx<T> load_elem(const void* base, int64_t off) {
   if constexpr (C) {
-    auto v = reinterpret_cast<const typename Elem<T, true>::type*>(base)[off];
+    auto v = gptr<typename Elem<T, true>::type>(base)[off];
     return Cplx<T>{v.x, v.y};
   } else {
-    return Cplx<T>{reinterpret_cast<const T*>(base)[off], T(0)};
+    return Cplx<T>{gptr<T>(base)[off], T(0)};
   }
 }
 
@@ -47,9 +65,9 @@ __device__ inline void store_elem(void* base, int64_t off, Cplx<T> v) {
     typename Elem<T, true>::type w;
     w.x = v.re;
     w.y = v.im;
-    reinterpret_cast<typename Elem<T, true>::type*>(base)[off] = w;
+    gptr<typename Elem<T, true>::type>(base)[off] = w;
   } else {
-    reinterpret_cast<T*>(base)[off] = v.re;
+    gptr<T>(base)[off] = v.re;
   }
 }
 
@@ -59,9 +77,9 @@ __device__ inline void store_elem_nt(void* base, int64_t off, Cplx<T> v) {
   if constexpr (C) {
     typedef T v2 __attribute__((ext_vector_type(2)));
     v2 w = {v.re, v.im};
-    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(base) + off);
+    __builtin_nontemporal_store(w, gptr<v2>(base) + off);
   } else {
-    __builtin_nontemporal_store(v.re, reinterpret_cast<T*>(base) + off);
+    __builtin_nontemporal_store(v.re, gptr<T>(base) + off);
   }
 }
 

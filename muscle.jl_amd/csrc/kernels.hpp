@@ -44,11 +44,19 @@ struct OffTab {
   FastDiv div;      // divisor P
   uint32_t base;    // added to every index (batch chunks)
   uint32_t single;  // one-level table: off(i) = lo[i]
+  // The tables are device (global) memory: read as such they are global, not
+  // flat, loads (a pending flat access makes every later wait a full drain).
   __device__ inline int64_t operator[](uint32_t i) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(1))) int64_t* gp;
+    const gp l = (gp)lo, h = (gp)hi;
+#else
+    const int64_t *l = lo, *h = hi;
+#endif
     i += base;
-    if (single) return lo[i];
+    if (single) return l[i];
     const uint32_t q = div.div(i);
-    return lo[i - q * div.d] + hi[q];
+    return l[i - q * div.d] + h[q];
   }
 };
 
