@@ -518,11 +518,6 @@ __device__ __forceinline__ void shallow_compute(const ShallowItem& s, const Cplx
   }
 }
 
-// Outputs of a partial tile that lie outside C go here, so that every thread
-// issues the same number of stores whatever the tile: the wait for the next
-// item's gathered operands can then leave this item's stores in flight.
-__device__ double2 shallow_sink[SNT];
-
 template <typename T, bool CP, bool CM, bool CS>
 __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_t* soff, const Cplx<T>* acc) {
   const int tid = threadIdx.x;
@@ -532,11 +527,11 @@ __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_
 #pragma unroll
   for (int i = 0; i < SOUT; ++i) {
     const int m = CM ? f : v0 + vs * i, n = CM ? v0 + vs * i : f;
-    const bool in = m < s.mr && n < s.nr;
-    void* const base = in ? s.C : (void*)shallow_sink;
-    const int64_t o = in ? offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n] : (int64_t)tid;
-    if constexpr (CS) store_elem_nt<T, CP>(base, o, acc[i]);
-    else store_elem<T, CP>(base, o, acc[i]);
+    if (m < s.mr && n < s.nr) {
+      const int64_t o = offC + soff[SOFF_ROWC + m] + soff[SOFF_COLC + n];
+      if constexpr (CS) store_elem_nt<T, CP>(s.C, o, acc[i]);
+      else store_elem<T, CP>(s.C, o, acc[i]);
+    }
   }
 }
 
@@ -546,9 +541,8 @@ __device__ __forceinline__ void shallow_store(const ShallowItem& s, const int64_
 // workgroup that loops over items issues the next item's table phase and
 // first gather before the current item's stores, so its loads are not queued
 // behind them (vector memory operations complete in order), and stashes the
-// gathered pass right after the stores with a wait that leaves the SOUT
-// stores in flight.  soff is double-buffered (the stores read the current
-// item's C offsets after the next item's table phase).
+// gathered pass after the stores.  soff is double-buffered (the stores read
+// the current item's C offsets after the next item's table phase).
 template <typename T, bool CP, bool CM, bool CS>
 __global__ __launch_bounds__(SNT) void shallow_kernel(
     ShallowArgs a) {
@@ -587,8 +581,6 @@ __global__ __launch_bounds__(SNT) void shallow_kernel(
     }
     shallow_store<T, CP, CM, CS>(cur, soff[b], acc);
     if (!more) break;
-    // the stash waits for the gathered operands only: they were issued
-    // before the stores, and every thread issues SOUT stores
     shallow_stash<T>(nxt, sab, ra, rb);
     cur = nxt;
     b ^= 1;
