@@ -588,6 +588,30 @@ __global__ __launch_bounds__(SNT) void shallow_kernel(
   }
 }
 
+// One item per workgroup (launches of at most kShallowGrid items): nothing is
+// held across the stores, so the complex kernels fit in 72 registers and
+// three workgroups share a CU, where the looping kernel's 122 allow two
+// (real FP64 keeps its 84: capped at 80 it spilled).
+template <typename T, bool CP, bool CM, bool CS>
+__global__ __launch_bounds__(SNT, CP ? 6 : 4) void shallow_one_kernel(ShallowArgs a) {
+  __shared__ Cplx<T> sab[SMAXE * SKC];
+  __shared__ int64_t soff[SOFF_N];
+  const ShallowItem cur = shallow_tables(a, blockIdx.x, soff);
+  __syncthreads();
+  Cplx<T> acc[SOUT];
+#pragma unroll
+  for (int i = 0; i < SOUT; ++i) acc[i] = Cplx<T>{T(0), T(0)};
+  for (int k0 = 0; k0 < cur.K; k0 += SKC) {
+    Cplx<T> ra[SGA], rb[SGA];
+    shallow_gather<T, CP>(cur, soff, k0, ra, rb);
+    if (k0 > 0) __syncthreads();  // the readers of sab are done
+    shallow_stash<T>(cur, sab, ra, rb);
+    __syncthreads();
+    shallow_compute<T, CP, CM>(cur, sab, min(SKC, cur.K - k0), acc);
+  }
+  shallow_store<T, CP, CM, CS>(cur, soff, acc);
+}
+
 // ---- skinny pairs ---------------------------------------------------------
 // A pair whose output has a short side of S <= kSkinnyMaxS elements does at
 // most S complex products per element of the long side's operand it reads,
@@ -792,9 +816,15 @@ size_t batch_lds_bytes(bool dbl, bool cplx, int tile) {
 void launch_shallow(bool dbl, bool cplx, bool cm, bool cs, const ShallowArgs& a, hipStream_t s) {
   if (a.nitems <= 0) return;
   const unsigned grid = (unsigned)std::min<int64_t>(a.nitems, kShallowGrid);
-#define MH_SHALLOW_CS(T, CP, CM)                                                                    \
-  if (cs) hipLaunchKernelGGL((shallow_kernel<T, CP, CM, true>), dim3(grid), dim3(SNT), 0, s, a);   \
-  else hipLaunchKernelGGL((shallow_kernel<T, CP, CM, false>), dim3(grid), dim3(SNT), 0, s, a);
+#define MH_SHALLOW_CS(T, CP, CM)                                                                     \
+  if (a.nitems <= kShallowGrid) {                                                                   \
+    if (cs) hipLaunchKernelGGL((shallow_one_kernel<T, CP, CM, true>), dim3(grid), dim3(SNT), 0, s, a); \
+    else hipLaunchKernelGGL((shallow_one_kernel<T, CP, CM, false>), dim3(grid), dim3(SNT), 0, s, a);   \
+  } else if (cs) {                                                                                  \
+    hipLaunchKernelGGL((shallow_kernel<T, CP, CM, true>), dim3(grid), dim3(SNT), 0, s, a);          \
+  } else {                                                                                          \
+    hipLaunchKernelGGL((shallow_kernel<T, CP, CM, false>), dim3(grid), dim3(SNT), 0, s, a);         \
+  }
 #define MH_SHALLOW(T, CP)       \
   if (cm) {                     \
     MH_SHALLOW_CS(T, CP, true)  \
