@@ -259,6 +259,41 @@ def test_shallow_pairs(T, tol, tile):
     batch.close()
 
 
+def test_shallow_launch_beyond_one_tile_per_workgroup():
+    """A shallow launch of more than 2^20 tiles runs the looping kernel (a
+    workgroup takes a second tile, whose offsets and first K pass it loads
+    before storing the first): 65 Float32 pairs of 8192 x 8192 outputs with
+    K = 4 (1,064,960 tiles of one C orientation) match a plain torch fp32 sum
+    of the K outer products (elementwise, no library GEMM and its precision
+    modes)."""
+    g = torch.Generator(device="cuda").manual_seed(23)
+    n, k, npairs = 8192, 4, 65
+    As = [torch.randn(n, k, device="cuda", generator=g) for _ in range(5)]
+    Bs = [torch.randn(k, n, device="cuda", generator=g) for _ in range(7)]
+    pairs = []
+    for p in range(npairs):
+        a = muscle.Tensor(As[p % 5].t().contiguous().t(), ["i", "j"])  # column-major, as Julia holds it
+        b = muscle.Tensor(Bs[p % 7].t().contiguous().t(), ["j", "k"])
+        c = muscle.Tensor(muscle.empty_col_major([n, n], torch.float32, torch.device("cuda")), ["i", "k"])
+        pairs.append((c, a, b))
+    batch = ContractionBatch(pairs, tiles=["64x64"] * npairs)
+    info = batch.info()
+    assert info["shallow_items"] == npairs * (n // 64) ** 2 > 1 << 20
+    for rep in range(2):
+        for c, a, b in pairs:
+            c.parent().fill_(float("nan"))
+        batch.run()
+        torch.cuda.synchronize()
+        for p, (c, a, b) in enumerate(pairs):
+            A, B = As[p % 5], Bs[p % 7]
+            ref = A[:, :1] * B[:1, :]
+            for j in range(1, k):
+                ref += A[:, j:j + 1] * B[j:j + 1, :]
+            err = (torch.linalg.norm(c.parent() - ref) / torch.linalg.norm(ref)).item()
+            assert err < 1e-6, (rep, p, err)
+    batch.close()
+
+
 SPLIT = [
     (["i", "j"], (8, 4096), ["j", "k"], (4096, 4), ["i", "k"]),  # tiny output, long K
     (["i", "j", "h"], (70, 1500, 1), ["j", "k", "h"], (1500, 40, 1), ["h", "k", "i"]),  # ragged tiles
