@@ -4,9 +4,9 @@
 //
 // Thousands of uneven pairwise contractions (the pairs of a contraction
 // tree) run in one launch: a fixed grid of workgroups (2 per CU) pulls
-// output tiles from a global work queue (one atomic per tile), largest pairs
-// first, until the queue is drained -- every workgroup reaches the exit test
-// with the same item number.  Each tile is a gather-GEMM tile like the plain
+// output tiles from work queues (one per XCD in a large launch, one atomic per
+// tile), largest pairs first, until every queue is drained -- every workgroup
+// reaches the exit test with no item left.  Each tile is a gather-GEMM tile like the plain
 // kernel's (offset tables into LDS, MFMA 16x16x4, scatter to C), with a tile
 // shape chosen per pair (64x64, 32x128 or 128x32) so skinny contractions do
 // not waste the workgroup.  The 64 x 64 shape double-buffers its LDS tiles,

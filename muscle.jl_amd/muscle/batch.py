@@ -60,8 +60,8 @@ class ContractionBatch:
         return inf.as_dict()
 
     def pair_info(self, i: int) -> dict:
-        """Route (empty, tiles, split_k, shallow or solo), tile, size class, K slices and
-        fused shape of pair i."""
+        """Route (empty, tiles, split_k, shallow, solo or skinny), tile, size class, K
+        slices and fused shape of pair i."""
         inf = _lib.BatchPairInfo()
         _lib.check(_lib.lib.muscle_hip_batch_get_pair_info(self._b, int(i), ctypes.byref(inf)))
         out = inf.as_dict()
