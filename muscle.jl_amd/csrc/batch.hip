@@ -90,6 +90,10 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
   constexpr int LA = BM + LDP, LB = BN + LDP;
   constexpr int STAGE = PL * BK * (LA + LB);  // elements of one LDS buffer
+  // LDS column of row (operand index) m at k is m ^ k, a permutation inside
+  // each aligned 16-block: a K-contiguous operand's 16 lanes of one m then
+  // store to 16 different banks (rows LA apart all start on one bank), and
+  // the MFMA reads of one k stay conflict-free.
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int m0 = tm * BM, n0 = tn * BN;
@@ -114,6 +118,7 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
     const int m = m0 + a_mm[i];
     a_ok[i] = m < M;
     a_row[i] = pr.rowA[a_ok[i] ? m : M - 1] + offA0;
+    a_mm[i] ^= a_kk[i];  // from here on: the swizzled LDS column
   }
 #pragma unroll
   for (int i = 0; i < NEB; ++i) {
@@ -128,6 +133,17 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
     const int n = n0 + b_nn[i];
     b_ok[i] = n < N;
     b_col[i] = pr.colB[b_ok[i] ? n : N - 1] + offB0;
+    b_nn[i] ^= b_kk[i];
+  }
+  // this lane's MFMA reads at K step ks (rows k = 4 ks + (lane >> 4)): the
+  // swizzled column (c ^ k) with c = lane & 15 is (c ^ (lane >> 4)) ^ 4 ks, so
+  // row and column together are one lane offset per ks plus constants
+  int xa[BK / 4], xb[BK / 4];
+#pragma unroll
+  for (int ks = 0; ks < BK / 4; ++ks) {
+    const int x = ((lane & 15) ^ (lane >> 4)) ^ (ks << 2);
+    xa[ks] = (lane >> 4) * LA + x;
+    xb[ks] = (lane >> 4) * LB + x;
   }
   Cplx<T> ra[NEA], rb[NEB];
   acc_t acc0[TM][TN], acc1[TM][TN], acc2[TM][TN];
@@ -169,17 +185,16 @@ __device__ __forceinline__ void batch_tile(const BatchPair& pr, int bz, int tm, 
     const T* As = lds + (buf) * STAGE;                                           \
     const T* Bs = As + PL * BK * LA;                                             \
     _Pragma("unroll") for (int ks = 0; ks < BK / 4; ++ks) {                      \
-      const int kr = ks * 4 + (lane >> 4);                                       \
       T ar[TM], ai[TM], br[TN], bi[TN];                                          \
       _Pragma("unroll") for (int i = 0; i < TM; ++i) {                           \
-        const int mm = wm * (BM / WM) + i * 16 + (lane & 15);                    \
-        ar[i] = As[kr * LA + mm];                                                \
-        ai[i] = CP ? As[(BK + kr) * LA + mm] : T(0);                             \
+        const int o = xa[ks] + ks * 4 * LA + wm * (BM / WM) + i * 16;            \
+        ar[i] = As[o];                                                           \
+        ai[i] = CP ? As[BK * LA + o] : T(0);                                     \
       }                                                                          \
       _Pragma("unroll") for (int j = 0; j < TN; ++j) {                           \
-        const int nn = wn * (BN / WN) + j * 16 + (lane & 15);                    \
-        br[j] = Bs[kr * LB + nn];                                                \
-        bi[j] = CP ? Bs[(BK + kr) * LB + nn] : T(0);                             \
+        const int o = xb[ks] + ks * 4 * LB + wn * (BN / WN) + j * 16;            \
+        br[j] = Bs[o];                                                           \
+        bi[j] = CP ? Bs[BK * LB + o] : T(0);                                     \
       }                                                                          \
       _Pragma("unroll") for (int i = 0; i < TM; ++i)                             \
         _Pragma("unroll") for (int j = 0; j < TN; ++j) {                         \
