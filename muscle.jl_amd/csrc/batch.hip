@@ -606,7 +606,7 @@ __global__ __launch_bounds__(SNT) void shallow_kernel(
 // One item per workgroup (launches of at most kShallowGrid items): nothing is
 // held across the stores, so the complex kernels fit in 72 registers and
 // three workgroups share a CU, where the looping kernel's 122 allow two
-// (real FP64 keeps its 84: capped at 80 it spilled).
+// (real FP64 takes 96: capped at 80 it spilled).
 template <typename T, bool CP, bool CM, bool CS>
 __global__ __launch_bounds__(SNT, CP ? 6 : 4) void shallow_one_kernel(ShallowArgs a) {
   __shared__ Cplx<T> sab[SMAXE * SKC];
